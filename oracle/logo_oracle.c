@@ -1,0 +1,459 @@
+/*
+ * logo_oracle.c -- CPU restatement of LOGO's 2-opt local-search path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker; never linked by the product).
+ * Each function cites the reference file:line whose behaviour it restates.
+ * Arithmetic is exact integer EUC_2D; see logo_oracle.h for what differs
+ * from the reference on purpose.
+ */
+#include "logo_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* PRNG (build-defined; the reference seeds libc rand() from the clock,      */
+/* common.c:253-259)                                                          */
+/* ------------------------------------------------------------------------ */
+static uint64_t splitmix64(uint64_t *x)
+{
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void orc_rng_seed(orc_rng_t *r, uint64_t seed)
+{
+    uint64_t x = seed;
+    for (int k = 0; k < 4; k++) r->s[k] = splitmix64(&x);
+}
+
+static inline uint64_t rotl64(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+
+uint64_t orc_rng_next(orc_rng_t *r)
+{
+    uint64_t *s = r->s;
+    uint64_t result = rotl64(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return result;
+}
+
+double orc_rng_u01(orc_rng_t *r) { return (double)(orc_rng_next(r) >> 11) * 0x1.0p-53; }
+
+/* ------------------------------------------------------------------------ */
+/* Workloads                                                                  */
+/* ------------------------------------------------------------------------ */
+void orc_gen_uniform(uint32_t n, uint32_t side, uint64_t seed, int32_t *xy)
+{
+    orc_rng_t r;
+    orc_rng_seed(&r, seed);
+    for (uint32_t k = 0; k < n; k++) {
+        xy[2 * k] = (int32_t)(orc_rng_next(&r) % side);
+        xy[2 * k + 1] = (int32_t)(orc_rng_next(&r) % side);
+    }
+}
+
+static int32_t clamp_coord(double v, uint32_t side)
+{
+    double r = nearbyint(v); /* round half to even, default FP environment */
+    if (r < 0.0) return 0;
+    if (r > (double)(side - 1)) return (int32_t)(side - 1);
+    return (int32_t)r;
+}
+
+/* Clustered instance: nclusters centres uniform in [0,side)^2; the first
+ * n - round(noise_frac*n) points go round-robin to the centres with a
+ * Box-Muller Gaussian offset of sigma_frac*side per axis, rounded and clamped;
+ * the rest are uniform noise points. */
+void orc_gen_clustered(uint32_t n, uint32_t side, uint32_t nclusters,
+                       double sigma_frac, double noise_frac, uint64_t seed,
+                       int32_t *xy)
+{
+    orc_rng_t r;
+    orc_rng_seed(&r, seed);
+    double *cx = (double *)malloc(sizeof(double) * nclusters);
+    double *cy = (double *)malloc(sizeof(double) * nclusters);
+    for (uint32_t c = 0; c < nclusters; c++) {
+        cx[c] = orc_rng_u01(&r) * (double)side;
+        cy[c] = orc_rng_u01(&r) * (double)side;
+    }
+    uint32_t nnoise = (uint32_t)nearbyint(noise_frac * (double)n);
+    if (nnoise > n) nnoise = n;
+    uint32_t nclust = n - nnoise;
+    double sigma = sigma_frac * (double)side;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint32_t k = 0; k < nclust; k++) {
+        uint32_t c = k % nclusters;
+        double u1 = ((double)(orc_rng_next(&r) >> 11) + 1.0) * 0x1.0p-53; /* (0,1] */
+        double u2 = orc_rng_u01(&r);
+        double rad = sqrt(-2.0 * log(u1));
+        double ang = two_pi * u2;
+        double gx = rad * cos(ang);
+        double gy = rad * sin(ang);
+        xy[2 * k] = clamp_coord(cx[c] + sigma * gx, side);
+        xy[2 * k + 1] = clamp_coord(cy[c] + sigma * gy, side);
+    }
+    for (uint32_t k = nclust; k < n; k++) {
+        xy[2 * k] = (int32_t)(orc_rng_next(&r) % side);
+        xy[2 * k + 1] = (int32_t)(orc_rng_next(&r) % side);
+    }
+    free(cx);
+    free(cy);
+}
+
+/* Seeded Fisher-Yates; the reference uses std::random_shuffle
+ * (construction.c:129-137). */
+void orc_random_perm(uint32_t n, uint64_t seed, uint32_t *route)
+{
+    orc_rng_t r;
+    orc_rng_seed(&r, seed);
+    for (uint32_t k = 0; k < n; k++) route[k] = k;
+    for (uint32_t k = n - 1; k >= 1; k--) {
+        uint32_t j = (uint32_t)(orc_rng_next(&r) % ((uint64_t)k + 1));
+        uint32_t t = route[k];
+        route[k] = route[j];
+        route[j] = t;
+    }
+}
+
+/* Greedy nearest neighbour (the reference builds NN with a kd-tree that drops
+ * duplicate points, kdtree_utils.c:279-301; this one is exact and keeps all
+ * points). */
+void orc_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route)
+{
+    unsigned char *used = (unsigned char *)calloc(n, 1);
+    uint32_t cur = start;
+    used[cur] = 1;
+    route[0] = cur;
+    for (uint32_t k = 1; k < n; k++) {
+        int64_t bx = xy[2 * cur], by = xy[2 * cur + 1];
+        uint64_t best = UINT64_MAX;
+        uint32_t arg = 0;
+        for (uint32_t u = 0; u < n; u++) {
+            if (used[u]) continue;
+            int64_t dx = xy[2 * u] - bx, dy = xy[2 * u + 1] - by;
+            uint64_t s = (uint64_t)(dx * dx + dy * dy);
+            if (s < best) { best = s; arg = u; }
+        }
+        used[arg] = 1;
+        route[k] = arg;
+        cur = arg;
+    }
+    free(used);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Distances                                                                  */
+/* ------------------------------------------------------------------------ */
+static inline uint64_t isqrt_u64(uint64_t s)
+{
+    uint64_t k = (uint64_t)sqrt((double)s);
+    while (k * k > s) k--;
+    while ((k + 1) * (k + 1) <= s) k++;
+    return k;
+}
+
+/* TSPLIB EUC_2D nint(sqrt(s)) for integer s.  sqrt(s) is never k + 1/2 for
+ * integer s, so nint(sqrt(s)) = k + [s > k^2 + k] exactly, k = isqrt(s).
+ * The reference's float version is headers.h:126-142. */
+uint32_t orc_nint_sqrt(uint64_t s)
+{
+    uint64_t k = isqrt_u64(s);
+    return (uint32_t)(k + (s > k * k + k ? 1u : 0u));
+}
+
+static inline uint32_t dist_xy(int64_t x1, int64_t y1, int64_t x2, int64_t y2)
+{
+    int64_t dx = x1 - x2, dy = y1 - y2;
+    return orc_nint_sqrt((uint64_t)(dx * dx + dy * dy));
+}
+
+void orc_nint_sqrt_batch(const uint64_t *s, uint32_t *out, uint64_t count)
+{
+    for (uint64_t k = 0; k < count; k++) out[k] = orc_nint_sqrt(s[k]);
+}
+
+uint32_t orc_dist(const int32_t *xy, uint32_t a, uint32_t b)
+{
+    return dist_xy(xy[2 * a], xy[2 * a + 1], xy[2 * b], xy[2 * b + 1]);
+}
+
+/* headers.h:170-189: sum of consecutive edges plus the closing edge. */
+uint64_t orc_route_length(uint32_t n, const int32_t *xy, const uint32_t *route)
+{
+    uint64_t d = 0;
+    for (uint32_t k = 1; k < n; k++) d += orc_dist(xy, route[k - 1], route[k]);
+    if (n > 0) d += orc_dist(xy, route[n - 1], route[0]);
+    return d;
+}
+
+/* construction.c:152-183 (returns an error code instead of exit(-1)). */
+int orc_check_tour(uint32_t n, const uint32_t *route)
+{
+    if (n < 3) return -1;
+    unsigned char *seen = (unsigned char *)calloc(n, 1);
+    int ok = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        if (route[k] >= n || seen[route[k]]) { ok = -1; break; }
+        seen[route[k]] = 1;
+    }
+    free(seen);
+    return ok;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sweep                                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t n;
+    const int64_t *px, *py; /* tour-ordered coordinates, cpuSolver.cpp:67-79 */
+    const uint32_t *dk;     /* dk[k] = d(P[k-1],P[k]) for k >= 1            */
+    uint32_t i_lo, i_hi;    /* i in [i_lo, i_hi)                            */
+    int32_t best;
+    uint32_t bi, bj;
+} sweep_job;
+
+/* cpuSolver.cpp:104-146: i ascending, J ascending, update on strict '<',
+ * only negative gains are recorded. */
+static void *sweep_range(void *arg)
+{
+    sweep_job *jb = (sweep_job *)arg;
+    const int64_t *px = jb->px, *py = jb->py;
+    const uint32_t *dk = jb->dk;
+    uint32_t n = jb->n;
+    int32_t best = 0;
+    uint32_t bi = 0, bj = 0;
+    for (uint32_t i = jb->i_lo; i < jb->i_hi; i++) {
+        int64_t xi = px[i], yi = py[i], xh = px[i - 1], yh = py[i - 1];
+        int32_t c1 = (int32_t)dk[i];
+        for (uint32_t J = i + 2; J <= n - 1; J++) {
+            int32_t g = (int32_t)dist_xy(xi, yi, px[J], py[J]) +
+                        (int32_t)dist_xy(xh, yh, px[J - 1], py[J - 1]) - c1 -
+                        (int32_t)dk[J];
+            if (g < best) {
+                best = g;
+                bi = i;
+                bj = J;
+            }
+        }
+    }
+    jb->best = best;
+    jb->bi = bi;
+    jb->bj = bj;
+    return NULL;
+}
+
+int orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
+             orc_best2_out *out)
+{
+    out->i = 0;
+    out->j = 0;
+    out->minchange = 0;
+    if (n < 4) return 0;
+    int64_t *px = (int64_t *)malloc(sizeof(int64_t) * n);
+    int64_t *py = (int64_t *)malloc(sizeof(int64_t) * n);
+    uint32_t *dk = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (uint32_t k = 0; k < n; k++) {
+        px[k] = xy[2 * route[k]];
+        py[k] = xy[2 * route[k] + 1];
+    }
+    dk[0] = dist_xy(px[n - 1], py[n - 1], px[0], py[0]);
+    for (uint32_t k = 1; k < n; k++) dk[k] = dist_xy(px[k - 1], py[k - 1], px[k], py[k]);
+
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    uint32_t rows = n - 3; /* i in [1, n-3] */
+    if ((uint32_t)threads > rows) threads = (int)rows;
+    sweep_job jobs[256];
+    pthread_t tid[256];
+    /* split the triangle into ranges of roughly equal pair counts
+     * (cpuMTSolver.cpp:109-138 splits the pair range per thread) */
+    double total = (double)(n - 2) * (double)(n - 3) / 2.0;
+    uint32_t lo = 1;
+    for (int t = 0; t < threads; t++) {
+        uint32_t hi;
+        if (t == threads - 1) {
+            hi = n - 2;
+        } else {
+            double target = total * (double)(t + 1) / (double)threads, acc = 0.0;
+            /* pairs with i < h: sum_{i=1}^{h-1} (n-2-i) */
+            hi = lo;
+            while (hi < n - 2) {
+                double h = (double)hi;
+                acc = (h - 1.0) * (double)(n - 2) - (h - 1.0) * h / 2.0;
+                if (acc >= target) break;
+                hi++;
+            }
+        }
+        jobs[t].n = n;
+        jobs[t].px = px;
+        jobs[t].py = py;
+        jobs[t].dk = dk;
+        jobs[t].i_lo = lo;
+        jobs[t].i_hi = hi;
+        lo = hi;
+    }
+    if (threads == 1) {
+        sweep_range(&jobs[0]);
+    } else {
+        for (int t = 0; t < threads; t++) pthread_create(&tid[t], NULL, sweep_range, &jobs[t]);
+        for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    }
+    /* merge in ascending i order with strict '<': identical to one scan */
+    int32_t best = 0;
+    uint32_t bi = 0, bj = 0;
+    for (int t = 0; t < threads; t++) {
+        if (jobs[t].best < best) {
+            best = jobs[t].best;
+            bi = jobs[t].bi;
+            bj = jobs[t].bj;
+        }
+    }
+    free(px);
+    free(py);
+    free(dk);
+    if (best < 0) {
+        out->i = bi;
+        out->j = bj;
+        out->minchange = best;
+        return 1;
+    }
+    return 0;
+}
+
+/* twoOptLocalSolver.cpp:41-48 in closed form:
+ *   rotate(begin, begin+i, end); reverse(begin+(J-i), end)
+ * gives r' = r[i..J-1] ++ r[i-1], ..., r[0] ++ r[n-1], ..., r[J]. */
+void orc_apply(uint32_t n, uint32_t *route, uint32_t i, uint32_t j)
+{
+    if (i > j) { uint32_t t = i; i = j; j = t; }
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t w = 0;
+    for (uint32_t k = i; k < j; k++) tmp[w++] = route[k];
+    for (uint32_t k = i; k-- > 0;) tmp[w++] = route[k];
+    for (uint32_t k = n; k-- > j;) tmp[w++] = route[k];
+    memcpy(route, tmp, sizeof(uint32_t) * n);
+    free(tmp);
+}
+
+/* twoOptLocalSolver.cpp:22-64: step, apply while the gain is negative. */
+uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
+                          int threads, uint64_t max_moves,
+                          orc_best2_out *log, uint64_t log_cap,
+                          uint64_t *sweeps, uint64_t *length)
+{
+    uint64_t moves = 0, sw = 0;
+    uint64_t L = orc_route_length(n, xy, route);
+    for (;;) {
+        if (max_moves && moves >= max_moves) break;
+        orc_best2_out o;
+        sw++;
+        if (!orc_step(n, xy, route, threads, &o)) break;
+        orc_apply(n, route, o.i, o.j);
+        L = (uint64_t)((int64_t)L + o.minchange);
+        if (log && moves < log_cap) log[moves] = o;
+        moves++;
+    }
+    if (sweeps) *sweeps = sw;
+    if (length) *length = L;
+    return moves;
+}
+
+/* Build-defined kick: three distinct cut points from 1..n-1, sorted. */
+void orc_double_bridge(uint32_t n, uint32_t *route, orc_rng_t *rng, uint32_t *cuts)
+{
+    uint32_t p[3];
+    int have = 0;
+    while (have < 3) {
+        uint32_t v = 1u + (uint32_t)(orc_rng_next(rng) % (uint64_t)(n - 1));
+        int dup = 0;
+        for (int k = 0; k < have; k++) dup |= (p[k] == v);
+        if (!dup) p[have++] = v;
+    }
+    /* sort three values */
+    if (p[0] > p[1]) { uint32_t t = p[0]; p[0] = p[1]; p[1] = t; }
+    if (p[1] > p[2]) { uint32_t t = p[1]; p[1] = p[2]; p[2] = t; }
+    if (p[0] > p[1]) { uint32_t t = p[0]; p[0] = p[1]; p[1] = t; }
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t w = 0;
+    for (uint32_t k = 0; k < p[0]; k++) tmp[w++] = route[k];    /* A */
+    for (uint32_t k = p[1]; k < p[2]; k++) tmp[w++] = route[k]; /* C */
+    for (uint32_t k = p[0]; k < p[1]; k++) tmp[w++] = route[k]; /* B */
+    for (uint32_t k = p[2]; k < n; k++) tmp[w++] = route[k];    /* D */
+    memcpy(route, tmp, sizeof(uint32_t) * n);
+    free(tmp);
+    if (cuts) { cuts[0] = p[0]; cuts[1] = p[1]; cuts[2] = p[2]; }
+}
+
+/* ILSGlobalSolver.cpp:46-47: rotate(begin, begin+a, end);
+ * reverse(begin+(b-a), end), for a <= b. */
+void orc_rotate_reverse(uint32_t n, uint32_t *route, uint32_t a, uint32_t b)
+{
+    if (b < a) { uint32_t t = a; a = b; b = t; }
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (uint32_t k = 0; k < n; k++) tmp[k] = route[(a + k) % n];
+    uint32_t lo = b - a, hi = n - 1;
+    while (lo < hi) {
+        uint32_t t = tmp[lo];
+        tmp[lo] = tmp[hi];
+        tmp[hi] = t;
+        lo++;
+        hi--;
+    }
+    memcpy(route, tmp, sizeof(uint32_t) * n);
+    free(tmp);
+}
+
+/* ILSGlobalSolver.cpp:37-45: a, b = rand() % n, sorted (here from the seeded
+ * generator), then the rotate+reverse above. */
+void orc_ref_kick(uint32_t n, uint32_t *route, orc_rng_t *rng)
+{
+    uint32_t a = (uint32_t)(orc_rng_next(rng) % n);
+    uint32_t b = (uint32_t)(orc_rng_next(rng) % n);
+    orc_rotate_reverse(n, route, a, b);
+}
+
+/* ILSGlobalSolverSequential.cpp:60-100 with a fixed round count in place of
+ * the target/time stop rule: LO, accept iff strictly shorter else revert to
+ * the best, kick the best. */
+uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                 uint32_t rounds, int kick_kind, int threads,
+                 uint64_t *round_lengths, uint64_t *total_moves)
+{
+    orc_rng_t rng;
+    orc_rng_seed(&rng, seed);
+    uint64_t L, tm = 0;
+    tm += orc_local_search(n, xy, route, threads, 0, NULL, 0, NULL, &L);
+    uint64_t Lbest = L;
+    uint32_t *best = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    memcpy(best, route, sizeof(uint32_t) * n);
+    if (round_lengths) round_lengths[0] = Lbest;
+    for (uint32_t r = 0; r < rounds; r++) {
+        memcpy(cur, best, sizeof(uint32_t) * n);
+        if (kick_kind == ORC_KICK_REFERENCE)
+            orc_ref_kick(n, cur, &rng);
+        else
+            orc_double_bridge(n, cur, &rng, NULL);
+        tm += orc_local_search(n, xy, cur, threads, 0, NULL, 0, NULL, &L);
+        if (L < Lbest) {
+            Lbest = L;
+            memcpy(best, cur, sizeof(uint32_t) * n);
+        }
+        if (round_lengths) round_lengths[r + 1] = Lbest;
+    }
+    memcpy(route, best, sizeof(uint32_t) * n);
+    free(best);
+    free(cur);
+    if (total_moves) *total_moves = tm;
+    return Lbest;
+}

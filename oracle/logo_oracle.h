@@ -1,0 +1,117 @@
+/*
+ * logo_oracle.h -- CPU restatement of LOGO's 2-opt local-search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in logo-tsp-solver_amd/.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it.  The product never links,
+ * loads or calls it.
+ *
+ * Semantics restated (reference = krocki/LOGO-TSP-Solver v0.62):
+ *   - neighbourhood i in [1,n-3], J in [i+2,n-1]     src/solvers/cpuSolver.cpp:104,123
+ *   - gain d(P[i],P[J]) + d(P[i-1],P[J-1])
+ *          - d(P[i-1],P[i]) - d(P[J-1],P[J])          src/solvers/cpuSolver.cpp:133-138
+ *   - strict '<' scan => lexicographically lowest (i,J)  src/solvers/cpuSolver.cpp:140-145
+ *   - accept iff gain < 0, apply rotate+reverse       src/solvers/twoOptLocalSolver.cpp:41-48
+ *   - loop until no improving move                    src/solvers/twoOptLocalSolver.cpp:33-63
+ *   - tour length incl. closing edge, unsigned long   src/headers.h:170-189
+ *   - ILS accept-if-strictly-shorter / revert         src/solvers/ILSGlobalSolverSequential.cpp:73-99
+ *   - random rotate+reverse kick                      src/solvers/ILSGlobalSolver.cpp:37-48
+ *   - tour validity                                   src/utils/construction.c:152-183
+ *
+ * Deliberate differences from the reference (its arithmetic is not exact):
+ *   - every edge length is the exact TSPLIB EUC_2D value nint(sqrt(dx^2+dy^2))
+ *     on integer coordinates (the reference mixes float sqrtf, +0.5f and
+ *     truncation, headers.h:126-142, cpuSolver.cpp:133-138);
+ *   - randomness comes from xoshiro256** seeded by SplitMix64 instead of
+ *     libc rand() (common.c:253-259), so runs are reproducible;
+ *   - the ILS kick used by the configs is a seeded double-bridge.
+ *
+ * Pins (see tests/test_oracle.py): the nint interval structure over all
+ * s < 2^24, Python math.isqrt golden vectors for s < 2^46, the
+ * mona-lisa100K tour length 5,757,191 stated by the reference's own tour
+ * file, lattice closed forms, brute-force enumeration on tiny tours,
+ * std::rotate/std::reverse for the apply step, and the reference's own
+ * CPUSolver built from its sources (oracle/_ref) on integer-distance inputs.
+ */
+#ifndef LOGO_ORACLE_H
+#define LOGO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- PRNG: SplitMix64-seeded xoshiro256** ---------------------------- */
+typedef struct { uint64_t s[4]; } orc_rng_t;
+void     orc_rng_seed(orc_rng_t *r, uint64_t seed);
+uint64_t orc_rng_next(orc_rng_t *r);
+/* uniform double in [0,1): top 53 bits */
+double   orc_rng_u01(orc_rng_t *r);
+
+/* ---- synthetic workloads (coordinates are interleaved int32 x,y) ----- */
+void orc_gen_uniform(uint32_t n, uint32_t side, uint64_t seed, int32_t *xy);
+void orc_gen_clustered(uint32_t n, uint32_t side, uint32_t nclusters,
+                       double sigma_frac, double noise_frac, uint64_t seed,
+                       int32_t *xy);
+void orc_random_perm(uint32_t n, uint64_t seed, uint32_t *route);
+/* greedy nearest neighbour from `start`, exact int64 squared distance,
+ * lowest city id on ties; O(n^2) */
+void orc_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route);
+
+/* ---- distances -------------------------------------------------------- */
+/* nint(sqrt(s)) = k + [s > k^2 + k] with k = isqrt(s) */
+uint32_t orc_nint_sqrt(uint64_t s);
+void     orc_nint_sqrt_batch(const uint64_t *s, uint32_t *out, uint64_t count);
+uint32_t orc_dist(const int32_t *xy, uint32_t a, uint32_t b);
+uint64_t orc_route_length(uint32_t n, const int32_t *xy, const uint32_t *route);
+/* 0 if route is a permutation of 0..n-1 with n >= 3, else -1 */
+int      orc_check_tour(uint32_t n, const uint32_t *route);
+
+/* ---- 2-opt ------------------------------------------------------------ */
+/* same layout as the reference's best2_out (defs.h:181-187) */
+typedef struct { uint32_t i; uint32_t j; int32_t minchange; } orc_best2_out;
+
+/* One best-improvement sweep over the current tour.  Returns 1 and fills
+ * *out with the lexicographically lowest (i,J) among the minimal gains if an
+ * improving move exists, else returns 0 with out->minchange = 0.  threads > 1
+ * partitions the i-range like cpuMTSolver.cpp:109-138 and merges by the same
+ * key, so the result does not depend on the thread count. */
+int  orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
+              orc_best2_out *out);
+/* r' = r[i..J-1] ++ r[i-1..0] ++ r[n-1..J]  (i < J); twoOptLocalSolver.cpp:41-48 */
+void orc_apply(uint32_t n, uint32_t *route, uint32_t i, uint32_t j);
+
+/* Best-improvement local search to a 2-opt local optimum.  log (optional)
+ * receives the applied moves in order, at most log_cap of them.  max_moves = 0
+ * means unbounded.  Returns the number of applied moves. */
+uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
+                          int threads, uint64_t max_moves,
+                          orc_best2_out *log, uint64_t log_cap,
+                          uint64_t *sweeps, uint64_t *length);
+
+/* Double-bridge A B C D -> A C B D with 1 <= p1 < p2 < p3 <= n-1 drawn from
+ * the generator.  cuts (optional) receives p1,p2,p3. */
+void orc_double_bridge(uint32_t n, uint32_t *route, orc_rng_t *rng, uint32_t *cuts);
+/* rotate left by min(a,b), then reverse from |b-a| to the end
+ * (ILSGlobalSolver.cpp:46-47) */
+void orc_rotate_reverse(uint32_t n, uint32_t *route, uint32_t a, uint32_t b);
+/* Reference kick: a,b = rand()%n twice, sorted, rotate(a) then reverse from
+ * b-a (ILSGlobalSolver.cpp:37-48), with the seeded generator. */
+void orc_ref_kick(uint32_t n, uint32_t *route, orc_rng_t *rng);
+
+#define ORC_KICK_DOUBLE_BRIDGE 0
+#define ORC_KICK_REFERENCE     1
+
+/* ILS: LO(route) -> best; then `rounds` times: cur = kick(best); LO(cur);
+ * accept iff strictly shorter.  route receives the best tour.
+ * round_lengths (optional, rounds+1 entries) receives the best length after
+ * the initial LO and after every round.  Returns the best length. */
+uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                 uint32_t rounds, int kick_kind, int threads,
+                 uint64_t *round_lengths, uint64_t *total_moves);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
