@@ -1,0 +1,211 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X 2-opt local-search path (one GPU).
+
+Headline (`value`): 2-opt move evaluations per second of a whole best-
+improvement local search on the GPU, on the 33,810-city clustered instance
+(60 Gaussian clusters with sigma = 2% of the side plus 10% uniform noise in
+[0,10^6)^2, greedy nearest-neighbour start) run to a 2-opt local optimum.
+One "step" is one full local search from the same start tour; inputs are
+resident in HBM before the timed region (the 135 KB start tour is re-sent per
+step, as TwoOptLocalSolver::optimize's caller hands it over).
+
+Also reported on the same line:
+  roofline      sweep kernel, 14 algorithmic FP32 FLOP per evaluated move,
+                average kernel duration from HIP events on the engine stream
+                over one timed local search (same trajectory as the steps)
+  cpu_baseline  the reference's own CPU sweep (CPUSolver::optimizeStep built
+                from its sources, oracle/_ref) timed on this host on a bounded
+                sample of sweeps of the same instance and start tour
+  batched       2,048 instances with n log-uniform in [100,5000], coords
+                U{0..9999}^2, random starts, all to a local optimum in one
+                engine (batched sweeps, graph-replayed steps) versus the plain
+                one-instance-per-launch path (host round trip per move) on an
+                every-16th subset; final lengths cross-checked between the two
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "logo-tsp-solver_amd"))
+
+import logo2opt as L  # noqa: E402
+
+METRIC = "2-opt move evals/sec + instances/sec batched vs plain, bit-exact tour length"
+FLOP_PER_EVAL = 14        # 2 new edges x (2 sub + mul + fma(2) + sqrt + add)
+REF_FLOP_PER_EVAL = 36    # the reference's GPU GFLOPS convention (solver.c:305-306)
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pairs(n):
+    return (n - 2) * (n - 3) // 2
+
+
+def config2(seed=2):
+    n = 33810
+    xy = L.gen_clustered(n, 10**6, seed)
+    r0 = L.nn_tour(xy, 0)
+    return n, xy, r0
+
+
+def cpu_baseline(n, xy, r0, budget_s):
+    """Bounded sample of the reference's CPU sweep on this host."""
+    ref_avx = os.path.join(ROOT, "oracle", "_ref", "libref2opt_avx.so")
+    ref_sc = os.path.join(ROOT, "oracle", "_ref", "libref2opt.so")
+    path = ref_avx if os.path.exists(ref_avx) else ref_sc if os.path.exists(ref_sc) else None
+    fxy = np.ascontiguousarray(xy.astype(np.float32))
+    if path:
+        R = C.CDLL(path)
+        R.ref_step.argtypes = [np.ctypeslib.ndpointer(np.float32), np.ctypeslib.ndpointer(np.uint32), C.c_uint32,
+                               np.ctypeslib.ndpointer(np.int32)]
+        out = np.zeros(3, np.int32)
+        sweeps, t0 = 0, time.perf_counter()
+        while True:
+            R.ref_step(fxy, r0, n, out)
+            sweeps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s or sweeps >= 50:
+                break
+        variant = "AVX build (USE_CPU_SIMD, USE_AVX)" if path == ref_avx else "scalar build"
+        return {"value": sweeps * pairs(n) / el, "unit": "evals/s", "cores": 1, "kind": "reference",
+                "sample": f"{sweeps} CPUSolver::optimizeStep sweeps ({el:.1f} s) of the 33,810-city clustered "
+                          f"instance at its NN start tour, reference sources {variant}, one thread as --mode=cpu"}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = len(os.sched_getaffinity(0))
+    sweeps, t0 = 0, time.perf_counter()
+    while True:
+        O.step(xy, r0, threads)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or sweeps >= 50:
+            break
+    return {"value": sweeps * pairs(n) / el, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{sweeps} exact oracle sweeps ({el:.1f} s) of the 33,810-city clustered instance"}
+
+
+def batched_vs_plain(count, plain_every, seed=4):
+    rs = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rs.uniform(np.log(100), np.log(5000), count))).astype(int)
+    insts = []
+    for k, n in enumerate(sizes):
+        xy = L.gen_uniform(int(n), 10**4, 10_000 + k)
+        insts.append((int(n), L.coords_of(xy), L.random_perm(int(n), 20_000 + k)))
+    budget = sum(L.Engine.bytes_for(n) for n, _, _ in insts) + (64 << 20)
+    with L.Engine(hbm_budget=budget, max_slots=count) as eng:
+        slots = [eng.add(c, r) for n, c, r in insts]
+        assert all(s is not None for s in slots)
+        t0 = time.perf_counter()
+        live = eng.run(sync_every=32)
+        tb = time.perf_counter() - t0
+        assert live == 0
+        st = eng.stats()
+        lengths = [eng.result(s)["length"] for s in slots]
+    subset = list(range(0, count, plain_every))
+    ctxs = [(k, L.Context(insts[k][1])) for k in subset]
+    tp = 0.0
+    mism = 0
+    for k, ctx in ctxs:
+        t0 = time.perf_counter()
+        out = ctx.optimize(insts[k][2], sync_every=1)
+        tp += time.perf_counter() - t0
+        mism += out["length"] != lengths[k]
+        ctx.close()
+    ib = count / tb
+    ip = len(subset) / tp
+    return {
+        "workload": f"{count} instances, n log-uniform in [100,5000] (seed {seed}), coords U{{0..9999}}^2, "
+                    "random starts, each to a 2-opt local optimum",
+        "instances": count, "batched_s": tb, "batched_instances_per_s": ib,
+        "batched_evals_per_s": st["evals"] / tb, "moves": st["moves"], "sweeps": st["sweeps"],
+        "plain_sample": len(subset), "plain_s": tp, "plain_instances_per_s": ip,
+        "speedup": ib / ip, "plain_vs_batched_length_mismatches": int(mism),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=2048)
+    ap.add_argument("--plain-every", type=int, default=16)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        sys.exit(2)
+
+    n, xy, r0 = config2()
+    ctx = L.Context(L.coords_of(xy))
+    log(f"config2 n={n}: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        ref = ctx.optimize(r0, sync_every=64)
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ctx.optimize(r0, sync_every=64)
+    dt = time.perf_counter() - t0
+    s1 = ctx.stats()
+    if args.warmup:
+        assert out["length"] == ref["length"] and out["moves"] == ref["moves"]
+    evals = s1["evals"] - s0["evals"]
+    sweeps_per = (s1["sweeps"] - s0["sweeps"]) // args.steps
+    fb_per = (s1["fallbacks"] - s0["fallbacks"]) // args.steps
+    log(f"timed {args.steps} local searches: {dt:.3f}s, {out['moves']} moves, {sweeps_per} sweeps each")
+
+    # roofline: the same local search again with every sweep launch event-timed
+    ctx.set_timing(True)
+    a0 = ctx.stats()
+    out_t = ctx.optimize(r0, sync_every=64)
+    a1 = ctx.stats()
+    ctx.set_timing(False)
+    assert out_t["length"] == out["length"]
+    launches = a1["sweep_launches"] - a0["sweep_launches"]
+    avg_ms = (a1["sweep_ms"] - a0["sweep_ms"]) / launches
+    work_sweeps = a1["sweeps"] - a0["sweeps"]
+    # algorithmic FLOP per launch averaged over launches (launches after
+    # convergence inside a sync batch do no work and are counted too)
+    flop_per_launch = FLOP_PER_EVAL * pairs(n) * work_sweeps / launches
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "kernel": "l2o::k_sweep<2,256,4>", "avg_kernel_ms": avg_ms, "launches": launches,
+                "evals_per_s_kernel": pairs(n) * work_sweeps / launches / (avg_ms * 1e-3),
+                "ref_convention_tflops": achieved * REF_FLOP_PER_EVAL / FLOP_PER_EVAL}
+    ctx.close()
+
+    res = {
+        "metric": METRIC, "value": evals / dt, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "33,810 clustered cities (60 Gaussian clusters, sigma 2% of side, + 10% uniform "
+                               "noise) in [0,1e6)^2, greedy NN start, best-improvement 2-opt to a local optimum",
+                   "n": n, "seed": 2, "moves": out["moves"], "sweeps": sweeps_per, "length": out["length"],
+                   "fallbacks_per_search": fb_per, "parallelism": "single GPU"},
+        "roofline": roofline,
+    }
+    if not args.skip_cpu:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline(n, xy, r0, args.cpu_budget)
+    if not args.skip_batch:
+        log("batched vs plain")
+        res["batched"] = batched_vs_plain(args.batch_count, args.plain_every)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

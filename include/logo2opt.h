@@ -1,0 +1,208 @@
+/*
+ * logo2opt.h -- C ABI of the MI355X (gfx950) 2-opt local-search library.
+ *
+ * Drop-in boundary for the GPU 2-opt step of krocki/LOGO-TSP-Solver.  The
+ * reference reaches its GPU backend through the LocalSolver virtual interface
+ * (src/solvers/localSolver.h:116-125) implemented by CUDASolver
+ * (src/solvers/CUDASolver.h:33-120, src/solvers/CUDASolver.cpp:48-434).
+ * The entry points below replace that class method by method; the C++
+ * adapter in logo-tsp-solver_amd/csrc/hip_local_solver.h maps them back onto
+ * the reference's method names.
+ *
+ * Conventions
+ *   - Plain C structs with the reference's layouts: logo2opt_city_coords is
+ *     city_coords (src/defs.h:173-178), logo2opt_best2_out is best2_out
+ *     (src/defs.h:181-187), logo2opt_process_time is process_time
+ *     (src/defs.h:190-198).
+ *   - Coordinates must be integral with x and y spans below 2^22 (they are
+ *     translated so the minimum is 0); edge lengths are the exact TSPLIB
+ *     EUC_2D nint(sqrt(dx^2+dy^2)).  Tours are permutations of 0..n-1,
+ *     3 <= n <= 2^20.
+ *   - A move (i,J) names tour positions, 1 <= i, i+2 <= J <= n-1.  It is
+ *     applied as rotate+reverse: r' = r[i..J-1] ++ r[i-1..0] ++ r[n-1..J]
+ *     (src/solvers/twoOptLocalSolver.cpp:41-48).  Among the minimal gains the
+ *     lexicographically lowest (i,J) wins (src/solvers/cpuSolver.cpp:140-145).
+ *   - Every call returns 0 or a negative LOGO2OPT_E* code; nothing exits or
+ *     throws across the ABI (the reference calls exit(-1) on CUDA errors,
+ *     src/utils/UtilsCUDA.cpp:318-328).
+ *   - A context or engine is confined to one host thread at a time, owns its
+ *     own HIP stream and keeps its device buffers resident between calls.
+ *     Host arrays stay owned by the caller; the library copies them.
+ */
+#ifndef LOGO2OPT_H
+#define LOGO2OPT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference data layouts ------------------------------------------- */
+typedef struct { float x; float y; } logo2opt_city_coords;           /* defs.h:173-178 */
+typedef struct { uint32_t i; uint32_t j; int32_t minchange; } logo2opt_best2_out; /* defs.h:181-187 */
+typedef struct {                                                      /* defs.h:190-198 */
+    double kernel_time;
+    double HtD_time;
+    double DtH_time;
+    logo2opt_best2_out out;
+} logo2opt_process_time;
+
+/* ---- error codes ------------------------------------------------------- */
+#define LOGO2OPT_OK          0
+#define LOGO2OPT_EINVAL     -1  /* bad argument (null pointer, n out of range) */
+#define LOGO2OPT_ECOORD     -2  /* coordinates not integral or span >= 2^22    */
+#define LOGO2OPT_ETOUR      -3  /* route is not a permutation of 0..n-1        */
+#define LOGO2OPT_ENOMEM     -4  /* host or device allocation failed           */
+#define LOGO2OPT_EDEVICE    -5  /* a HIP runtime call failed                   */
+#define LOGO2OPT_EAGAIN     -6  /* engine full: HBM budget or slot limit       */
+#define LOGO2OPT_ESTATE     -7  /* call not valid in the current state         */
+#define LOGO2OPT_ENOTFOUND  -8  /* unknown slot / request id                   */
+const char *logo2opt_strerror(int code);
+
+#define LOGO2OPT_MAX_N (1u << 20)
+#define LOGO2OPT_KICK_DOUBLE_BRIDGE 0 /* build-defined seeded double-bridge        */
+#define LOGO2OPT_KICK_REFERENCE     1 /* rotate+reverse, ILSGlobalSolver.cpp:37-48 */
+
+/* ======================================================================== */
+/* Single-instance context: the CUDASolver replacement                      */
+/* ======================================================================== */
+typedef struct logo2opt_ctx logo2opt_ctx;
+
+/* CUDASolver::init(dev) (CUDASolver.cpp:52-139): select the device, allocate
+ * resident buffers and copy the coordinates once. */
+int logo2opt_create(int device, uint32_t n, const logo2opt_city_coords *coords,
+                    logo2opt_ctx **out);
+/* CUDASolver::close / cleanCUDA (CUDASolver.cpp:141-144,388-401). */
+int logo2opt_destroy(logo2opt_ctx *ctx);
+
+/* CUDASolver::optimizeStep (CUDASolver.cpp:285-386): one best-improvement
+ * sweep of `route`, no apply.  out->minchange = 0 (and i = j = 0) when no
+ * improving move exists; otherwise out->i < out->j.  t (optional) receives the
+ * HtD / kernel / DtH times in seconds like CUDASolver::benchmark
+ * (CUDASolver.cpp:146-283), measured with HIP events. */
+int logo2opt_step(logo2opt_ctx *ctx, const uint32_t *route, logo2opt_best2_out *out,
+                  logo2opt_process_time *t);
+
+/* TwoOptLocalSolver::optimize (twoOptLocalSolver.cpp:22-64) run on the
+ * device: sweep, apply on the device, repeat to a local optimum.
+ * route_inout: start tour in, local optimum out.  length (optional): final
+ * uint64 length.  moves (optional): applied moves.  log (optional): the first
+ * log_cap applied moves (i, J, gain) in order.  sync_every = 1 reproduces the
+ * reference's one-round-trip-per-move loop; sync_every = K > 1 replays K
+ * sweep+apply steps as one captured HIP graph between host checks. */
+int logo2opt_optimize(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t *length,
+                      uint64_t *moves, logo2opt_best2_out *log, uint64_t log_cap,
+                      uint32_t sync_every);
+
+/* ILSGlobalSolverSequential::optimize (ILSGlobalSolverSequential.cpp:60-100)
+ * with a fixed round count: LO, then `rounds` times kick(best) -> LO ->
+ * accept iff strictly shorter.  Everything stays on the device.  route_inout
+ * receives the best tour; round_lengths (optional, rounds+1 entries) the best
+ * length after the first LO and after every round. */
+int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds,
+                 int kick, uint64_t *best_length, uint64_t *round_lengths);
+
+/* routeLength (src/headers.h:170-189), exact, computed on the device. */
+int logo2opt_route_length(logo2opt_ctx *ctx, const uint32_t *route, uint64_t *length);
+
+typedef struct {
+    uint64_t sweeps;        /* sweeps since create (moves + 1 per local search)  */
+    uint64_t moves;         /* applied 2-opt moves                                */
+    uint64_t evals;         /* candidate moves evaluated, (n-2)(n-3)/2 per sweep  */
+    uint64_t fallbacks;     /* candidates re-evaluated on the exact FP64 path     */
+    double   sweep_ms;      /* device time of timed sweep kernels (HIP events)    */
+    double   wall_ms;       /* host wall time inside optimize/ils/run             */
+    uint64_t sweep_launches;/* sweep kernel launches (timed mode: all are timed)  */
+} logo2opt_stats;
+int logo2opt_get_stats(logo2opt_ctx *ctx, logo2opt_stats *st);
+
+/* Timing mode (on != 0): every sweep launch of optimize/ils is bracketed by
+ * HIP events on the context's stream and summed into stats.sweep_ms; HIP
+ * graphs are not used while it is on. */
+int logo2opt_set_timing(logo2opt_ctx *ctx, int on);
+
+/* Times `sweeps` back-to-back launches of the sweep kernel on `route`
+ * (no apply) with HIP events on the context's stream: avg_ms receives the
+ * average kernel duration.  Used for the roofline figure. */
+int logo2opt_time_sweep(logo2opt_ctx *ctx, const uint32_t *route, uint32_t sweeps,
+                        double *avg_ms);
+
+/* Accuracy of the FP32 square root the sweep kernel uses (v_sqrt_f32): the
+ * largest distance in ulps from the correctly rounded result over the float
+ * bit patterns [lo_bits, hi_bits).  The sweep's exactness bound assumes at
+ * most 3 ulps; the test suite checks it over [1, 2^46).  No reference
+ * counterpart (the reference relies on --use_fast_math sqrtf, Makefile:84). */
+int logo2opt_sqrt_ulp_check(int device, uint32_t lo_bits, uint32_t hi_bits, uint32_t *max_ulp);
+
+/* ======================================================================== */
+/* Engine: many resident instances, batched sweeps, device ILS               */
+/* ======================================================================== */
+typedef struct logo2opt_engine logo2opt_engine;
+
+typedef struct {
+    uint32_t ils_rounds;   /* 0: local search only; >0: ILS with this many kicks */
+    uint32_t kick;         /* LOGO2OPT_KICK_*                                     */
+    uint64_t seed;         /* kick generator seed                                 */
+    uint32_t log_cap;      /* moves to log (0 = none)                             */
+    uint32_t reserved;
+} logo2opt_job;
+
+typedef struct {
+    uint64_t length;       /* final length (best length for ILS)                 */
+    uint64_t moves;        /* applied moves (all local searches)                 */
+    uint64_t sweeps;
+    uint64_t fallbacks;
+    uint32_t rounds_done;  /* ILS rounds completed                                */
+    uint32_t done;         /* 1 when finished                                     */
+} logo2opt_result;
+
+/* hbm_budget_bytes bounds the resident instance memory (0 = 1 GiB). */
+int logo2opt_engine_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots,
+                           logo2opt_engine **out);
+int logo2opt_engine_destroy(logo2opt_engine *e);
+/* Bytes of device memory an instance of n cities with `job` occupies. */
+uint64_t logo2opt_engine_bytes_for(uint32_t n, const logo2opt_job *job);
+/* Admit an instance: copies coordinates and start tour to the device.
+ * Returns LOGO2OPT_EAGAIN (nothing changed) when the HBM budget or the slot
+ * count is exhausted.  The new slot joins the next batched step. */
+int logo2opt_engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords,
+                        const uint32_t *route, const logo2opt_job *job, int32_t *slot);
+/* Run up to max_steps batched steps (each: one sweep of every active
+ * instance, the applies, the ILS bookkeeping), checking with the host every
+ * `sync_every` steps.  Returns when every slot is done or max_steps ran;
+ * *live receives the number of unfinished slots. */
+int logo2opt_engine_run(logo2opt_engine *e, uint32_t max_steps, uint32_t sync_every,
+                        uint32_t *live);
+/* Non-blocking status for a slot (reads device state). */
+int logo2opt_engine_result(logo2opt_engine *e, int32_t slot, uint32_t *route_out,
+                           logo2opt_result *res, logo2opt_best2_out *log);
+/* ILS slots: best length after the first local search and after each round
+ * completed so far (at most cap values). */
+int logo2opt_engine_round_lengths(logo2opt_engine *e, int32_t slot, uint64_t *out, uint32_t cap);
+/* Free a slot's device memory (finished or not). */
+int logo2opt_engine_remove(logo2opt_engine *e, int32_t slot);
+/* Slots finished since the last call (ids written to out, at most cap). */
+int logo2opt_engine_poll_done(logo2opt_engine *e, int32_t *out, uint32_t cap, uint32_t *count);
+int logo2opt_engine_stats(logo2opt_engine *e, logo2opt_stats *st);
+int logo2opt_engine_set_timing(logo2opt_engine *e, int on);
+
+/* ======================================================================== */
+/* Synthetic workloads and construction (host, deterministic)               */
+/* ======================================================================== */
+/* SplitMix64-seeded xoshiro256** streams; coordinates are interleaved int32. */
+void logo2opt_gen_uniform(uint32_t n, uint32_t side, uint64_t seed, int32_t *xy);
+void logo2opt_gen_clustered(uint32_t n, uint32_t side, uint32_t nclusters, double sigma_frac,
+                            double noise_frac, uint64_t seed, int32_t *xy);
+void logo2opt_random_perm(uint32_t n, uint64_t seed, uint32_t *route);
+/* Greedy nearest-neighbour tour from `start` (exact squared distances,
+ * lowest id on ties) on a uniform grid; replaces routeInit's NN option
+ * (construction.c:139-142, kdtree_utils.c:279-301). */
+int  logo2opt_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route);
+/* int32 xy -> reference city_coords */
+void logo2opt_xy_to_coords(uint32_t n, const int32_t *xy, logo2opt_city_coords *coords);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1063 @@
+// l2o_engine.hip -- host side of the C ABI declared in include/logo2opt.h.
+//
+// An engine owns one HIP stream, one HBM arena (the admission budget) and an
+// array of instance slots (l2o_inst, device resident).  A batched step is
+//   sweep  (one launch per tile shape, covering every live instance)
+//   apply  (one launch, gathers the winning move of every instance)
+//   ils    (one launch, accept/revert + kick for instances at a local optimum)
+// Steps are issued back to back; the host looks at the device only every
+// `sync_every` steps, and then through one 4-byte counter of live instances.
+// With sync_every > 1 the steps are captured once into a HIP graph and
+// replayed.  The single-instance context (the CUDASolver replacement,
+// CUDASolver.cpp:48-434) is an engine with one slot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/logo2opt.h"
+#include "l2o_launch.h"
+#include "l2o_state.h"
+
+using l2o::sweep_cfg;
+
+#define L2O_TRY(call)                                     \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return LOGO2OPT_EDEVICE;    \
+    } while (0)
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// tile-shape table per n bucket (written by the autotuner, tune.py)
+// ---------------------------------------------------------------------------
+struct tune_row {
+    uint32_t max_n;
+    sweep_cfg cfg;
+};
+
+std::mutex g_tune_mu;
+std::vector<tune_row> g_tune = {
+    {640, {1, 64, 1}},
+    {8192, {1, 128, 4}},
+    {LOGO2OPT_MAX_N, {2, 256, 4}},
+};
+bool g_tune_loaded = false;
+
+void load_tuning_locked()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *p = std::getenv("LOGO2OPT_TUNING");
+    std::string path = p ? p : "";
+    if (path.empty()) return;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    std::vector<tune_row> rows;
+    unsigned mx;
+    int k, t, w;
+    while (std::fscanf(f, "%u %d %d %d", &mx, &k, &t, &w) == 4) {
+        sweep_cfg c{k, t, w};
+        if (l2o::sweep_cfg_supported(c)) rows.push_back({mx, c});
+    }
+    std::fclose(f);
+    if (!rows.empty()) {
+        std::sort(rows.begin(), rows.end(), [](const tune_row &a, const tune_row &b) { return a.max_n < b.max_n; });
+        rows.back().max_n = LOGO2OPT_MAX_N;
+        g_tune = rows;
+    }
+}
+
+sweep_cfg cfg_for(uint32_t n)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tuning_locked();
+    for (const auto &r : g_tune)
+        if (n <= r.max_n) return r.cfg;
+    return g_tune.back().cfg;
+}
+
+// ---------------------------------------------------------------------------
+// first-fit arena over one hipMalloc (the HBM admission budget)
+// ---------------------------------------------------------------------------
+struct arena {
+    char *base = nullptr;
+    uint64_t size = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+
+    void init(char *b, uint64_t s)
+    {
+        base = b;
+        size = s;
+        free_.clear();
+        free_[0] = s;
+    }
+    bool alloc(uint64_t bytes, uint64_t *off)
+    {
+        bytes = (bytes + 255) & ~255ull;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const uint64_t rest = it->second - bytes;
+                const uint64_t o = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[o] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes)
+    {
+        bytes = (bytes + 255) & ~255ull;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint64_t free_bytes() const
+    {
+        uint64_t s = 0;
+        for (auto &kv : free_) s += kv.second;
+        return s;
+    }
+};
+
+// exact nint(sqrt(s)) on the host
+inline uint32_t nint_sqrt_u64(uint64_t s)
+{
+    uint64_t k = (uint64_t)std::sqrt((double)s);
+    while (k * k > s) --k;
+    while ((k + 1) * (k + 1) <= s) ++k;
+    return (uint32_t)(k + (s > k * k + k ? 1 : 0));
+}
+
+inline uint32_t dist_i(const int32_t *xy, uint32_t a, uint32_t b)
+{
+    const int64_t dx = (int64_t)xy[2 * a] - xy[2 * b], dy = (int64_t)xy[2 * a + 1] - xy[2 * b + 1];
+    return nint_sqrt_u64((uint64_t)(dx * dx + dy * dy));
+}
+
+uint64_t pairs_of(uint32_t n) { return n >= 4 ? (uint64_t)(n - 2) * (uint64_t)(n - 3) / 2 : 0; }
+
+struct slot_t {
+    bool used = false;
+    uint32_t n = 0;
+    uint64_t off = 0, bytes = 0;
+    logo2opt_job job{};
+    std::vector<int32_t> xy;  // translated integer coordinates
+    bool done = false;
+    bool done_reported = false;
+    float slack = 1.0f;
+};
+
+}  // namespace
+
+struct logo2opt_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char *d_arena = nullptr;
+    arena ar;
+    uint32_t max_slots = 0;
+    std::vector<slot_t> slots;
+    l2o_inst *d_insts = nullptr;
+    unsigned *d_live = nullptr;
+    unsigned *h_live = nullptr;  // pinned
+    unsigned live = 0;           // host mirror, valid between runs
+    unsigned long long *d_scratch = nullptr;
+    unsigned long long *h_scratch = nullptr;  // pinned
+
+    // work lists
+    struct group {
+        sweep_cfg cfg;
+        uint32_t off, count;
+    };
+    std::vector<group> groups;
+    l2o_sweep_item *d_sitems = nullptr;
+    size_t sitems_cap = 0;
+    l2o_apply_item *d_aitems = nullptr;
+    size_t aitems_cap = 0;
+    uint32_t n_aitems = 0;
+    uint32_t ils_grid = 0;
+    bool any_ils = false;
+    bool dirty = true;
+    unsigned live_at_build = 0;
+
+    hipGraphExec_t gexec = nullptr;
+    uint32_t gsteps = 0;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double wall_ms = 0.0, sweep_ms = 0.0;
+    uint64_t sweep_launches = 0;
+    std::vector<int32_t> done_queue;
+
+    // timing mode: every step's sweep launches are bracketed by HIP events on
+    // the engine stream (no graphs); elapsed times are summed at each sync
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    uint32_t tev_used = 0;
+};
+
+struct logo2opt_ctx {
+    logo2opt_engine *e = nullptr;
+    uint32_t n = 0;
+    std::vector<logo2opt_city_coords> coords;
+    int32_t slot = -1;
+    uint64_t sweeps = 0, moves = 0, evals = 0, fallbacks = 0;
+    double wall_ms = 0.0, sweep_ms = 0.0;
+};
+
+namespace {
+
+int check_coords(uint32_t n, const logo2opt_city_coords *c, std::vector<int32_t> &xy, float *slack)
+{
+    if (!c) return LOGO2OPT_EINVAL;
+    xy.resize(2 * (size_t)n);
+    double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+    for (uint32_t k = 0; k < n; ++k) {
+        const double x = c[k].x, y = c[k].y;
+        if (!std::isfinite(x) || !std::isfinite(y) || x != std::floor(x) || y != std::floor(y))
+            return LOGO2OPT_ECOORD;
+        mnx = std::min(mnx, x);
+        mny = std::min(mny, y);
+        mxx = std::max(mxx, x);
+        mxy = std::max(mxy, y);
+    }
+    const double lim = (double)(1u << 22);
+    if (mxx - mnx >= lim || mxy - mny >= lim) return LOGO2OPT_ECOORD;
+    for (uint32_t k = 0; k < n; ++k) {
+        xy[2 * k] = (int32_t)(c[k].x - mnx);
+        xy[2 * k + 1] = (int32_t)(c[k].y - mny);
+    }
+    // |exact gain - FP32 estimate| <= 1 (two nint roundings) + 2 e_d + rounding
+    // of the three FP32 adds and of the threshold, with e_d <= dmax * 2^-21
+    // (covers FP32 rounding of s and up to 3.5 ulp of v_sqrt_f32; tests check
+    // v_sqrt_f32 against the correctly rounded sqrt) and each add within
+    // ulp(4 dmax) <= dmax * 2^-21.  Total <= 1 + dmax * 2^-18; 2^-8 margin.
+    const double dmax = std::sqrt((mxx - mnx) * (mxx - mnx) + (mxy - mny) * (mxy - mny));
+    const double s = 1.0 + dmax * std::ldexp(1.0, -18) + std::ldexp(1.0, -8);
+    *slack = std::nextafter((float)s, INFINITY);
+    return LOGO2OPT_OK;
+}
+
+int check_tour(uint32_t n, const uint32_t *r)
+{
+    if (!r) return LOGO2OPT_EINVAL;
+    std::vector<unsigned char> seen(n, 0);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (r[k] >= n || seen[r[k]]) return LOGO2OPT_ETOUR;
+        seen[r[k]] = 1;
+    }
+    return LOGO2OPT_OK;
+}
+
+uint64_t bytes_for(uint32_t n, const logo2opt_job *job)
+{
+    auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+    uint64_t b = 2 * al(16ull * n) + 2 * al(4ull * n);
+    if (job && job->ils_rounds) b += al(16ull * n) + al(4ull * n) + al(8ull * ((uint64_t)job->ils_rounds + 1));
+    if (job && job->log_cap) b += al(12ull * job->log_cap);
+    return b;
+}
+
+// writes the ordered points / tour of `route` into slot buffers and the
+// instance record; the stream must be idle for this slot.
+int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint32_t phase)
+{
+    slot_t &s = e->slots[id];
+    const uint32_t n = s.n;
+    auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+    char *base = e->d_arena + s.off;
+    l2o_inst in;
+    std::memset(&in, 0, sizeof(in));
+    in.pts[0] = (float4 *)base;
+    in.pts[1] = (float4 *)(base + al(16ull * n));
+    in.tour[0] = (uint32_t *)(base + 2 * al(16ull * n));
+    in.tour[1] = (uint32_t *)(base + 2 * al(16ull * n) + al(4ull * n));
+    char *p = base + 2 * al(16ull * n) + 2 * al(4ull * n);
+    if (s.job.ils_rounds) {
+        in.bpts = (float4 *)p;
+        p += al(16ull * n);
+        in.btour = (uint32_t *)p;
+        p += al(4ull * n);
+        in.rlog = (unsigned long long *)p;
+        p += al(8ull * ((uint64_t)s.job.ils_rounds + 1));
+    }
+    if (s.job.log_cap) in.log = (l2o_move *)p;
+    in.n = n;
+    in.log_cap = s.job.log_cap;
+    in.slack = s.slack;
+    in.ils_rounds = s.job.ils_rounds;
+    in.kick = s.job.kick;
+    in.phase = phase;
+    in.key = L2O_SENTINEL;
+    // SplitMix64 -> xoshiro256** state (same seeding as the workload generators)
+    uint64_t x = s.job.seed;
+    for (int k = 0; k < 4; ++k) {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        in.rng[k] = z ^ (z >> 31);
+    }
+    std::vector<float4> pts(n);
+    uint64_t L = 0;
+    const int32_t *xy = s.xy.data();
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t c = route[k], pc = route[k == 0 ? n - 1 : k - 1];
+        const uint32_t d = dist_i(xy, pc, c);
+        L += d;
+        pts[k] = make_float4((float)xy[2 * c], (float)xy[2 * c + 1], (float)d, 0.0f);
+    }
+    in.length = L;
+    L2O_TRY(hipMemcpyAsync(in.pts[0], pts.data(), 16ull * n, hipMemcpyHostToDevice, e->stream));
+    L2O_TRY(hipMemcpyAsync(in.tour[0], route, 4ull * n, hipMemcpyHostToDevice, e->stream));
+    L2O_TRY(hipMemcpyAsync(e->d_insts + id, &in, sizeof(in), hipMemcpyHostToDevice, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    return LOGO2OPT_OK;
+}
+
+void drop_graph(logo2opt_engine *e)
+{
+    if (e->gexec) {
+        (void)hipGraphExecDestroy(e->gexec);
+        e->gexec = nullptr;
+    }
+    e->gsteps = 0;
+}
+
+int rebuild_lists(logo2opt_engine *e)
+{
+    std::map<std::tuple<int, int, int>, std::vector<l2o_sweep_item>> by_cfg;
+    std::vector<l2o_apply_item> ai;
+    e->ils_grid = 0;
+    e->any_ils = false;
+    for (uint32_t id = 0; id < e->max_slots; ++id) {
+        const slot_t &s = e->slots[id];
+        if (!s.used || s.done) continue;
+        const uint32_t n = s.n;
+        const sweep_cfg c = cfg_for(n);
+        auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves)];
+        for (uint32_t g0 = 2; g0 + 2 <= n; g0 += c.diag()) {
+            const uint32_t len = n - 1 - g0;
+            for (uint32_t a0 = 0; a0 < len; a0 += c.t) v.push_back({id, g0, a0, 0});
+        }
+        const uint32_t nch = (n + L2O_APPLY_CHUNK - 1) / L2O_APPLY_CHUNK;
+        for (uint32_t ch = 0; ch < nch; ++ch) ai.push_back({id, ch});
+        if (s.job.ils_rounds) {
+            e->any_ils = true;
+            e->ils_grid = id + 1;
+        }
+    }
+    std::vector<l2o_sweep_item> all;
+    e->groups.clear();
+    for (auto &kv : by_cfg) {
+        const sweep_cfg c{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
+        // XCD-aware order: workgroup b is dispatched to XCD b % 8, so give
+        // XCD x a contiguous share of the (instance-ordered) item list; its L2
+        // then holds only the points of its own instances and tiles.  Shares
+        // differ by at most one item, which keeps b % 8 aligned to the share.
+        const auto &v = kv.second;
+        const uint32_t cnt = (uint32_t)v.size();
+        const uint32_t X = 8, base = cnt / X, rem = cnt % X;
+        std::vector<l2o_sweep_item> perm;
+        perm.reserve(cnt);
+        for (uint32_t r = 0; r <= base; ++r)
+            for (uint32_t x = 0; x < X; ++x) {
+                if (r == base && x >= rem) break;
+                perm.push_back(v[x * base + std::min(x, rem) + r]);
+            }
+        e->groups.push_back({c, (uint32_t)all.size(), cnt});
+        all.insert(all.end(), perm.begin(), perm.end());
+    }
+    if (all.size() > e->sitems_cap) {
+        if (e->d_sitems) (void)hipFree(e->d_sitems);
+        e->sitems_cap = std::max<size_t>(all.size(), 2 * e->sitems_cap);
+        L2O_TRY(hipMalloc(&e->d_sitems, e->sitems_cap * sizeof(l2o_sweep_item)));
+    }
+    if (ai.size() > e->aitems_cap) {
+        if (e->d_aitems) (void)hipFree(e->d_aitems);
+        e->aitems_cap = std::max<size_t>(ai.size(), 2 * e->aitems_cap);
+        L2O_TRY(hipMalloc(&e->d_aitems, e->aitems_cap * sizeof(l2o_apply_item)));
+    }
+    if (!all.empty())
+        L2O_TRY(hipMemcpyAsync(e->d_sitems, all.data(), all.size() * sizeof(l2o_sweep_item), hipMemcpyHostToDevice,
+                               e->stream));
+    if (!ai.empty())
+        L2O_TRY(hipMemcpyAsync(e->d_aitems, ai.data(), ai.size() * sizeof(l2o_apply_item), hipMemcpyHostToDevice,
+                               e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    e->n_aitems = (uint32_t)ai.size();
+    e->dirty = false;
+    e->live_at_build = e->live;
+    drop_graph(e);
+    return LOGO2OPT_OK;
+}
+
+int issue_step(logo2opt_engine *e)
+{
+    const bool tm = e->timing && !e->groups.empty();
+    if (tm) {
+        if (e->tev_used + 2 > e->tev.size()) {
+            for (int q = 0; q < 64; ++q) {
+                hipEvent_t ev;
+                L2O_TRY(hipEventCreate(&ev));
+                e->tev.push_back(ev);
+            }
+        }
+        L2O_TRY(hipEventRecord(e->tev[e->tev_used], e->stream));
+    }
+    for (const auto &g : e->groups) {
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+        e->sweep_launches += 1;
+    }
+    if (tm) {
+        L2O_TRY(hipEventRecord(e->tev[e->tev_used + 1], e->stream));
+        e->tev_used += 2;
+    }
+    L2O_TRY(l2o::launch_apply(e->d_aitems, e->n_aitems, e->d_insts, e->d_live, e->stream));
+    if (e->any_ils) L2O_TRY(l2o::launch_ils(e->d_insts, e->ils_grid, e->d_live, e->stream));
+    return LOGO2OPT_OK;
+}
+
+// after a stream sync: add the timed sweep intervals
+int collect_timing(logo2opt_engine *e)
+{
+    for (uint32_t q = 0; q + 1 < e->tev_used; q += 2) {
+        float ms = 0.0f;
+        L2O_TRY(hipEventElapsedTime(&ms, e->tev[q], e->tev[q + 1]));
+        e->sweep_ms += ms;
+    }
+    e->tev_used = 0;
+    return LOGO2OPT_OK;
+}
+
+int issue_steps(logo2opt_engine *e, uint32_t k, uint32_t sync_every)
+{
+    if (sync_every > 1 && k == sync_every && !e->timing) {
+        if (!e->gexec || e->gsteps != k) {
+            drop_graph(e);
+            hipGraph_t g;
+            L2O_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            int rc = LOGO2OPT_OK;
+            for (uint32_t s = 0; s < k && rc == LOGO2OPT_OK; ++s) rc = issue_step(e);
+            hipError_t ce = hipStreamEndCapture(e->stream, &g);
+            if (rc != LOGO2OPT_OK) return rc;
+            L2O_TRY(ce);
+            hipError_t ie = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            L2O_TRY(ie);
+            e->gsteps = k;
+        }
+        L2O_TRY(hipGraphLaunch(e->gexec, e->stream));
+        return LOGO2OPT_OK;
+    }
+    for (uint32_t s = 0; s < k; ++s) {
+        int rc = issue_step(e);
+        if (rc) return rc;
+    }
+    return LOGO2OPT_OK;
+}
+
+// refresh host knowledge of finished slots
+int scan_done(logo2opt_engine *e)
+{
+    std::vector<l2o_inst> h(e->max_slots);
+    L2O_TRY(hipMemcpyAsync(h.data(), e->d_insts, sizeof(l2o_inst) * e->max_slots, hipMemcpyDeviceToHost, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    for (uint32_t id = 0; id < e->max_slots; ++id) {
+        slot_t &s = e->slots[id];
+        if (s.used && !s.done && h[id].phase == L2O_DONE) {
+            s.done = true;
+            e->done_queue.push_back((int32_t)id);
+        }
+    }
+    return LOGO2OPT_OK;
+}
+
+int engine_run(logo2opt_engine *e, uint32_t max_steps, uint32_t sync_every, uint32_t *live_out)
+{
+    if (sync_every == 0) sync_every = 1;
+    const double t0 = now_ms();
+    if (e->dirty) {
+        int rc = rebuild_lists(e);
+        if (rc) return rc;
+    }
+    uint32_t steps = 0;
+    while (steps < max_steps && e->live > 0) {
+        const uint32_t k = std::min(sync_every, max_steps - steps);
+        int rc = issue_steps(e, k, sync_every);
+        if (rc) return rc;
+        steps += k;
+        L2O_TRY(hipMemcpyAsync(e->h_live, e->d_live, sizeof(unsigned), hipMemcpyDeviceToHost, e->stream));
+        L2O_TRY(hipStreamSynchronize(e->stream));
+        if (e->timing) {
+            rc = collect_timing(e);
+            if (rc) return rc;
+        }
+        const unsigned lv = *e->h_live;
+        if (lv != e->live) {
+            e->live = lv;
+            rc = scan_done(e);
+            if (rc) return rc;
+            // drop finished instances from the work lists once they are a
+            // noticeable share of the launched workgroups
+            if (e->live == 0 || e->live * 4 < e->live_at_build * 3) {
+                rc = rebuild_lists(e);
+                if (rc) return rc;
+            }
+        }
+    }
+    e->wall_ms += now_ms() - t0;
+    if (live_out) *live_out = e->live;
+    return LOGO2OPT_OK;
+}
+
+int engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
+               const logo2opt_job *job, int32_t *slot_out)
+{
+    if (n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
+    slot_t tmp;
+    int rc = check_coords(n, coords, tmp.xy, &tmp.slack);
+    if (rc) return rc;
+    rc = check_tour(n, route);
+    if (rc) return rc;
+    int32_t id = -1;
+    for (uint32_t k = 0; k < e->max_slots; ++k)
+        if (!e->slots[k].used) {
+            id = (int32_t)k;
+            break;
+        }
+    if (id < 0) return LOGO2OPT_EAGAIN;
+    logo2opt_job j{};
+    if (job) j = *job;
+    if (j.kick > 1) return LOGO2OPT_EINVAL;
+    if (j.ils_rounds && n < 8) return LOGO2OPT_EINVAL;  // double-bridge needs room for 3 cuts
+    const uint64_t bytes = bytes_for(n, &j);
+    uint64_t off;
+    if (!e->ar.alloc(bytes, &off)) return LOGO2OPT_EAGAIN;
+    slot_t &s = e->slots[id];
+    s = std::move(tmp);
+    s.used = true;
+    s.n = n;
+    s.off = off;
+    s.bytes = bytes;
+    s.job = j;
+    s.done = false;
+    s.done_reported = false;
+    rc = upload_instance(e, id, route, L2O_LO);
+    if (rc) {
+        e->ar.release(off, bytes);
+        s = slot_t();
+        return rc;
+    }
+    e->live += 1;
+    *e->h_live = e->live;
+    L2O_TRY(hipMemcpyAsync(e->d_live, e->h_live, sizeof(unsigned), hipMemcpyHostToDevice, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    e->dirty = true;
+    *slot_out = id;
+    return LOGO2OPT_OK;
+}
+
+int engine_result(logo2opt_engine *e, int32_t id, uint32_t *route_out, logo2opt_result *res, logo2opt_best2_out *log)
+{
+    if (id < 0 || (uint32_t)id >= e->max_slots || !e->slots[id].used) return LOGO2OPT_ENOTFOUND;
+    l2o_inst in;
+    L2O_TRY(hipMemcpyAsync(&in, e->d_insts + id, sizeof(in), hipMemcpyDeviceToHost, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    const bool ils = in.ils_rounds > 0;
+    if (res) {
+        res->length = ils && in.have_best ? in.best_length : in.length;
+        res->moves = in.moves;
+        res->sweeps = in.sweeps;
+        res->fallbacks = in.fallbacks;
+        res->rounds_done = in.rounds_done;
+        res->done = in.phase == L2O_DONE;
+    }
+    if (route_out) {
+        const uint32_t *src = ils && in.have_best ? in.btour : in.tour[in.parity];
+        L2O_TRY(hipMemcpyAsync(route_out, src, 4ull * in.n, hipMemcpyDeviceToHost, e->stream));
+    }
+    if (log && in.log_cap) {
+        const uint64_t m = std::min<uint64_t>(in.moves, in.log_cap);
+        if (m) L2O_TRY(hipMemcpyAsync(log, in.log, 12ull * m, hipMemcpyDeviceToHost, e->stream));
+    }
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    return LOGO2OPT_OK;
+}
+
+int engine_remove(logo2opt_engine *e, int32_t id)
+{
+    if (id < 0 || (uint32_t)id >= e->max_slots || !e->slots[id].used) return LOGO2OPT_ENOTFOUND;
+    slot_t &s = e->slots[id];
+    if (!s.done) {
+        // retire a running instance: mark it done on the device first
+        l2o_inst in;
+        L2O_TRY(hipMemcpyAsync(&in, e->d_insts + id, sizeof(in), hipMemcpyDeviceToHost, e->stream));
+        L2O_TRY(hipStreamSynchronize(e->stream));
+        if (in.phase != L2O_DONE) {
+            e->live -= 1;
+            *e->h_live = e->live;
+            L2O_TRY(hipMemcpyAsync(e->d_live, e->h_live, sizeof(unsigned), hipMemcpyHostToDevice, e->stream));
+        }
+    }
+    l2o_inst blank;
+    std::memset(&blank, 0, sizeof(blank));
+    blank.phase = L2O_FREE;
+    L2O_TRY(hipMemcpyAsync(e->d_insts + id, &blank, sizeof(blank), hipMemcpyHostToDevice, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    e->ar.release(s.off, s.bytes);
+    s = slot_t();
+    e->dirty = true;
+    return LOGO2OPT_OK;
+}
+
+int engine_create(int device, uint64_t budget, uint32_t max_slots, logo2opt_engine **out)
+{
+    if (!out || max_slots == 0) return LOGO2OPT_EINVAL;
+    if (budget == 0) budget = 1ull << 30;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return LOGO2OPT_EDEVICE;
+    L2O_TRY(hipSetDevice(device));
+    logo2opt_engine *e = new (std::nothrow) logo2opt_engine();
+    if (!e) return LOGO2OPT_ENOMEM;
+    e->device = device;
+    e->max_slots = max_slots;
+    e->slots.resize(max_slots);
+    bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&e->d_arena, budget) == hipSuccess &&
+              hipMalloc(&e->d_insts, sizeof(l2o_inst) * max_slots) == hipSuccess &&
+              hipMalloc(&e->d_live, sizeof(unsigned)) == hipSuccess &&
+              hipMalloc(&e->d_scratch, sizeof(unsigned long long) * 4) == hipSuccess &&
+              hipHostMalloc(&e->h_live, sizeof(unsigned), hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc(&e->h_scratch, sizeof(unsigned long long) * 4, hipHostMallocDefault) == hipSuccess &&
+              hipEventCreate(&e->ev0) == hipSuccess && hipEventCreate(&e->ev1) == hipSuccess;
+    if (ok) {
+        e->ar.init(e->d_arena, budget);
+        ok = hipMemsetAsync(e->d_insts, 0, sizeof(l2o_inst) * max_slots, e->stream) == hipSuccess &&
+             hipMemsetAsync(e->d_live, 0, sizeof(unsigned), e->stream) == hipSuccess &&
+             hipStreamSynchronize(e->stream) == hipSuccess;
+    }
+    if (!ok) {
+        logo2opt_engine_destroy(e);
+        return LOGO2OPT_EDEVICE;
+    }
+    *out = e;
+    return LOGO2OPT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *logo2opt_strerror(int code)
+{
+    switch (code) {
+    case LOGO2OPT_OK: return "ok";
+    case LOGO2OPT_EINVAL: return "invalid argument";
+    case LOGO2OPT_ECOORD: return "coordinates must be integral with x/y spans below 2^22";
+    case LOGO2OPT_ETOUR: return "route is not a permutation of 0..n-1";
+    case LOGO2OPT_ENOMEM: return "out of memory";
+    case LOGO2OPT_EDEVICE: return "HIP runtime error";
+    case LOGO2OPT_EAGAIN: return "engine full (HBM budget or slots)";
+    case LOGO2OPT_ESTATE: return "invalid state for this call";
+    case LOGO2OPT_ENOTFOUND: return "unknown slot";
+    default: return "unknown error";
+    }
+}
+
+int logo2opt_engine_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots, logo2opt_engine **out)
+{
+    return engine_create(device, hbm_budget_bytes, max_slots, out);
+}
+
+int logo2opt_engine_destroy(logo2opt_engine *e)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    if (e->d_sitems) (void)hipFree(e->d_sitems);
+    if (e->d_aitems) (void)hipFree(e->d_aitems);
+    if (e->d_arena) (void)hipFree(e->d_arena);
+    if (e->d_insts) (void)hipFree(e->d_insts);
+    if (e->d_live) (void)hipFree(e->d_live);
+    if (e->d_scratch) (void)hipFree(e->d_scratch);
+    if (e->h_live) (void)hipHostFree(e->h_live);
+    if (e->h_scratch) (void)hipHostFree(e->h_scratch);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    for (hipEvent_t ev : e->tev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return LOGO2OPT_OK;
+}
+
+uint64_t logo2opt_engine_bytes_for(uint32_t n, const logo2opt_job *job) { return bytes_for(n, job); }
+
+int logo2opt_engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
+                        const logo2opt_job *job, int32_t *slot)
+{
+    if (!e || !slot) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    return engine_add(e, n, coords, route, job, slot);
+}
+
+int logo2opt_engine_run(logo2opt_engine *e, uint32_t max_steps, uint32_t sync_every, uint32_t *live)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    return engine_run(e, max_steps, sync_every, live);
+}
+
+int logo2opt_engine_result(logo2opt_engine *e, int32_t slot, uint32_t *route_out, logo2opt_result *res,
+                           logo2opt_best2_out *log)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    return engine_result(e, slot, route_out, res, log);
+}
+
+int logo2opt_engine_remove(logo2opt_engine *e, int32_t slot)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    return engine_remove(e, slot);
+}
+
+int logo2opt_engine_poll_done(logo2opt_engine *e, int32_t *out, uint32_t cap, uint32_t *count)
+{
+    if (!e || !count) return LOGO2OPT_EINVAL;
+    uint32_t c = 0;
+    auto &q = e->done_queue;
+    while (c < cap && c < q.size()) {
+        out[c] = q[c];
+        ++c;
+    }
+    q.erase(q.begin(), q.begin() + c);
+    *count = c;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_engine_stats(logo2opt_engine *e, logo2opt_stats *st)
+{
+    if (!e || !st) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    std::vector<l2o_inst> h(e->max_slots);
+    L2O_TRY(hipMemcpyAsync(h.data(), e->d_insts, sizeof(l2o_inst) * e->max_slots, hipMemcpyDeviceToHost, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    std::memset(st, 0, sizeof(*st));
+    for (uint32_t id = 0; id < e->max_slots; ++id) {
+        if (!e->slots[id].used) continue;
+        st->sweeps += h[id].sweeps;
+        st->moves += h[id].moves;
+        st->evals += h[id].sweeps * pairs_of(h[id].n);
+        st->fallbacks += h[id].fallbacks;
+    }
+    st->wall_ms = e->wall_ms;
+    st->sweep_ms = e->sweep_ms;
+    st->sweep_launches = e->sweep_launches;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_engine_set_timing(logo2opt_engine *e, int on)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    e->timing = on != 0;
+    drop_graph(e);
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_set_timing(logo2opt_ctx *ctx, int on)
+{
+    if (!ctx) return LOGO2OPT_EINVAL;
+    return logo2opt_engine_set_timing(ctx->e, on);
+}
+
+// ---------------------------------------------------------------------------
+// single-instance context
+// ---------------------------------------------------------------------------
+int logo2opt_create(int device, uint32_t n, const logo2opt_city_coords *coords, logo2opt_ctx **out)
+{
+    if (!out || !coords || n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
+    std::vector<int32_t> xy;
+    float slack;
+    int rc = check_coords(n, coords, xy, &slack);
+    if (rc) return rc;
+    logo2opt_ctx *c = new (std::nothrow) logo2opt_ctx();
+    if (!c) return LOGO2OPT_ENOMEM;
+    c->n = n;
+    c->coords.assign(coords, coords + n);
+    // room for one ILS instance with a modest move log
+    logo2opt_job j{};
+    j.ils_rounds = 1;
+    j.log_cap = 1u << 16;
+    const uint64_t budget = bytes_for(n, &j) + (1ull << 24);
+    rc = engine_create(device, budget, 1, &c->e);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_destroy(logo2opt_ctx *ctx)
+{
+    if (!ctx) return LOGO2OPT_EINVAL;
+    if (ctx->e) logo2opt_engine_destroy(ctx->e);
+    delete ctx;
+    return LOGO2OPT_OK;
+}
+
+namespace {
+
+void ctx_accumulate(logo2opt_ctx *c)
+{
+    logo2opt_stats st;
+    if (logo2opt_engine_stats(c->e, &st) == LOGO2OPT_OK) {
+        c->sweeps += st.sweeps;
+        c->moves += st.moves;
+        c->evals += st.evals;
+        c->fallbacks += st.fallbacks;
+    }
+}
+
+int ctx_load(logo2opt_ctx *c, const uint32_t *route, const logo2opt_job &job)
+{
+    if (c->slot >= 0) {
+        ctx_accumulate(c);
+        int rc = engine_remove(c->e, c->slot);
+        if (rc) return rc;
+        c->slot = -1;
+    }
+    return engine_add(c->e, c->n, c->coords.data(), route, &job, &c->slot);
+}
+
+}  // namespace
+
+int logo2opt_step(logo2opt_ctx *ctx, const uint32_t *route, logo2opt_best2_out *out, logo2opt_process_time *t)
+{
+    if (!ctx || !route || !out) return LOGO2OPT_EINVAL;
+    logo2opt_engine *e = ctx->e;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    const double h0 = now_ms();
+    logo2opt_job job{};
+    int rc = ctx_load(ctx, route, job);
+    if (rc) return rc;
+    rc = rebuild_lists(e);
+    if (rc) return rc;
+    const double h1 = now_ms();
+    L2O_TRY(hipEventRecord(e->ev0, e->stream));
+    for (const auto &g : e->groups)
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+    L2O_TRY(hipEventRecord(e->ev1, e->stream));
+    L2O_TRY(hipEventSynchronize(e->ev1));
+    float ms = 0.0f;
+    L2O_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    const double d0 = now_ms();
+    l2o_inst in;
+    L2O_TRY(hipMemcpyAsync(&in, e->d_insts + ctx->slot, sizeof(in), hipMemcpyDeviceToHost, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    const double d1 = now_ms();
+    if (in.key == L2O_SENTINEL) {
+        out->i = out->j = 0;
+        out->minchange = 0;
+    } else {
+        out->minchange = (int32_t)((long long)(in.key >> 40) - L2O_KEY_GAIN_BIAS);
+        out->i = (uint32_t)(in.key >> 20) & 0xFFFFFu;
+        out->j = (uint32_t)in.key & 0xFFFFFu;
+    }
+    ctx->sweep_ms += ms;
+    if (t) {
+        t->HtD_time = (h1 - h0) * 1e-3;
+        t->kernel_time = ms * 1e-3;
+        t->DtH_time = (d1 - d0) * 1e-3;
+        t->out = *out;
+    }
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_optimize(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t *length, uint64_t *moves,
+                      logo2opt_best2_out *log, uint64_t log_cap, uint32_t sync_every)
+{
+    if (!ctx || !route_inout) return LOGO2OPT_EINVAL;
+    logo2opt_engine *e = ctx->e;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    logo2opt_job job{};
+    job.log_cap = (uint32_t)std::min<uint64_t>(log ? log_cap : 0, 1u << 16);
+    int rc = ctx_load(ctx, route_inout, job);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    uint32_t live = 1;
+    rc = engine_run(e, 0xFFFFFFFFu, sync_every, &live);
+    if (rc) return rc;
+    ctx->wall_ms += now_ms() - t0;
+    logo2opt_result res;
+    rc = engine_result(e, ctx->slot, route_inout, &res, log);
+    if (rc) return rc;
+    if (length) *length = res.length;
+    if (moves) *moves = res.moves;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds, int kick,
+                 uint64_t *best_length, uint64_t *round_lengths)
+{
+    if (!ctx || !route_inout || (kick != 0 && kick != 1)) return LOGO2OPT_EINVAL;
+    if (rounds == 0) rounds = 0;
+    logo2opt_engine *e = ctx->e;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    logo2opt_job job{};
+    job.ils_rounds = rounds;
+    job.kick = (uint32_t)kick;
+    job.seed = seed;
+    if (rounds == 0) {
+        // a plain local search
+        uint64_t L = 0;
+        int rc = logo2opt_optimize(ctx, route_inout, &L, nullptr, nullptr, 0, 64);
+        if (best_length) *best_length = L;
+        if (round_lengths) round_lengths[0] = L;
+        return rc;
+    }
+    int rc = ctx_load(ctx, route_inout, job);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    uint32_t live = 1;
+    rc = engine_run(e, 0xFFFFFFFFu, 64, &live);
+    if (rc) return rc;
+    ctx->wall_ms += now_ms() - t0;
+    logo2opt_result res;
+    rc = engine_result(e, ctx->slot, route_inout, &res, nullptr);
+    if (rc) return rc;
+    if (best_length) *best_length = res.length;
+    if (round_lengths) {
+        rc = logo2opt_engine_round_lengths(e, ctx->slot, round_lengths, rounds + 1);
+        if (rc) return rc;
+    }
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_engine_round_lengths(logo2opt_engine *e, int32_t slot, uint64_t *out, uint32_t cap)
+{
+    if (!e || !out) return LOGO2OPT_EINVAL;
+    if (slot < 0 || (uint32_t)slot >= e->max_slots || !e->slots[slot].used) return LOGO2OPT_ENOTFOUND;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    l2o_inst in;
+    L2O_TRY(hipMemcpyAsync(&in, e->d_insts + slot, sizeof(in), hipMemcpyDeviceToHost, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    if (!in.rlog) return LOGO2OPT_ESTATE;
+    const uint32_t m = std::min<uint32_t>(cap, in.rounds_done + (in.phase == L2O_DONE ? 1u : 0u));
+    if (m) {
+        L2O_TRY(hipMemcpyAsync(out, in.rlog, 8ull * m, hipMemcpyDeviceToHost, e->stream));
+        L2O_TRY(hipStreamSynchronize(e->stream));
+    }
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_route_length(logo2opt_ctx *ctx, const uint32_t *route, uint64_t *length)
+{
+    if (!ctx || !route || !length) return LOGO2OPT_EINVAL;
+    logo2opt_engine *e = ctx->e;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    logo2opt_job job{};
+    int rc = ctx_load(ctx, route, job);
+    if (rc) return rc;
+    L2O_TRY(hipMemsetAsync(e->d_scratch, 0, sizeof(unsigned long long), e->stream));
+    L2O_TRY(l2o::launch_length(e->d_insts, (uint32_t)ctx->slot, ctx->n, e->d_scratch, e->stream));
+    L2O_TRY(hipMemcpyAsync(e->h_scratch, e->d_scratch, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    *length = e->h_scratch[0];
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_get_stats(logo2opt_ctx *ctx, logo2opt_stats *st)
+{
+    if (!ctx || !st) return LOGO2OPT_EINVAL;
+    logo2opt_stats cur;
+    std::memset(&cur, 0, sizeof(cur));
+    if (ctx->slot >= 0) {
+        int rc = logo2opt_engine_stats(ctx->e, &cur);
+        if (rc) return rc;
+    }
+    st->sweeps = ctx->sweeps + cur.sweeps;
+    st->moves = ctx->moves + cur.moves;
+    st->evals = ctx->evals + cur.evals;
+    st->fallbacks = ctx->fallbacks + cur.fallbacks;
+    st->sweep_ms = ctx->sweep_ms + ctx->e->sweep_ms;
+    st->wall_ms = ctx->wall_ms;
+    st->sweep_launches = ctx->e->sweep_launches;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_sqrt_ulp_check(int device, uint32_t lo_bits, uint32_t hi_bits, uint32_t *max_ulp)
+{
+    if (!max_ulp || lo_bits >= hi_bits) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    unsigned *d = nullptr;
+    L2O_TRY(hipMalloc(&d, sizeof(unsigned)));
+    hipError_t err = hipMemset(d, 0, sizeof(unsigned));
+    if (err == hipSuccess) err = l2o::launch_sqrt_ulp(lo_bits, hi_bits, d, nullptr);
+    unsigned h = 0;
+    if (err == hipSuccess) err = hipMemcpy(&h, d, sizeof(unsigned), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    L2O_TRY(err);
+    *max_ulp = h;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_time_sweep(logo2opt_ctx *ctx, const uint32_t *route, uint32_t sweeps, double *avg_ms)
+{
+    if (!ctx || !route || !avg_ms || sweeps == 0) return LOGO2OPT_EINVAL;
+    logo2opt_engine *e = ctx->e;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    logo2opt_job job{};
+    int rc = ctx_load(ctx, route, job);
+    if (rc) return rc;
+    rc = rebuild_lists(e);
+    if (rc) return rc;
+    // one untimed launch (code object load, caches)
+    for (const auto &g : e->groups)
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+    L2O_TRY(hipEventRecord(e->ev0, e->stream));
+    for (uint32_t s = 0; s < sweeps; ++s)
+        for (const auto &g : e->groups)
+            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+    L2O_TRY(hipEventRecord(e->ev1, e->stream));
+    L2O_TRY(hipEventSynchronize(e->ev1));
+    float ms = 0.0f;
+    L2O_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    *avg_ms = (double)ms / sweeps;
+    ctx->sweep_ms += ms;
+    return LOGO2OPT_OK;
+}
+
+}  // extern "C"

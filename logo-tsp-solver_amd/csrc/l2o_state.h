@@ -1,0 +1,83 @@
+// l2o_state.h -- device-resident instance state shared by the host engine and
+// the gfx950 kernels.
+//
+// Per instance (n cities) HBM holds, 16-byte aligned:
+//   pts[2][n]  float4 {x, y, D, 0}   tour-ordered coordinates (ping-pong);
+//                                     D[k] = exact d(P[k-1], P[k]), D[0] the
+//                                     closing edge d(P[n-1], P[0])
+//   tour[2][n] uint32                 city ids in the same order (ping-pong)
+//   bpts[n], btour[n]                 best tour of an ILS run
+//   log[log_cap]                      applied moves {i, J, gain}
+// The reference keeps the tour on the host and re-uploads ordered
+// coordinates every move (CUDASolver.cpp:316-329); here both stay resident and
+// the move is applied by a device gather.
+#pragma once
+#include <stdint.h>
+
+#define L2O_SENTINEL 0xFFFFFFFFFFFFFFFFull
+
+// key = ((gain + 2^24) << 40) | (i << 20) | J   for gain < 0, i < J < 2^20.
+// Unsigned order of keys is the lexicographic order of (gain, i, J), so the
+// minimum key is the reference's strict-'<' scan winner
+// (cpuSolver.cpp:140-145) and an atomicMin reduction is order independent.
+#define L2O_KEY_GAIN_BIAS (1ll << 24)
+
+enum l2o_phase : uint32_t {
+    L2O_FREE = 0,     // slot unused
+    L2O_LO = 1,       // local search running: sweep + apply every step
+    L2O_LO_DONE = 2,  // local optimum reached, ILS bookkeeping pending
+    L2O_DONE = 3,     // finished (result ready)
+};
+
+struct l2o_move {
+    uint32_t i;
+    uint32_t j;
+    int32_t gain;
+};
+
+struct l2o_inst {
+    // ---- fixed at admission
+    float4 *pts[2];
+    uint32_t *tour[2];
+    float4 *bpts;
+    uint32_t *btour;
+    l2o_move *log;
+    unsigned long long *rlog;  // ILS: best length after the first LO and each round
+    uint32_t n;
+    uint32_t log_cap;
+    float slack;          // bound on |exact gain - FP32 estimate| (see l2o_kernels.hip)
+    uint32_t ils_rounds;  // 0 = local search only
+    uint32_t kick;
+    uint32_t have_best;
+    // ---- dynamic
+    uint32_t phase;
+    uint32_t parity;
+    uint32_t apply_done;  // last-workgroup counter of the apply kernel
+    uint32_t rounds_done;
+    unsigned long long key;
+    unsigned long long length;
+    unsigned long long best_length;
+    unsigned long long moves;
+    unsigned long long sweeps;
+    unsigned long long fallbacks;
+    unsigned long long rng[4];
+};
+
+// One workgroup of the sweep kernel: diagonals g in [g0, g0 + DIAG) of the
+// (a, g = b - a) triangle, steps a in [a0, a0 + T).
+struct l2o_sweep_item {
+    uint32_t inst;
+    uint32_t g0;
+    uint32_t a0;
+    uint32_t pad;
+};
+
+// One workgroup of the apply kernel: elements [chunk*CHUNK, (chunk+1)*CHUNK).
+struct l2o_apply_item {
+    uint32_t inst;
+    uint32_t chunk;
+};
+
+#define L2O_APPLY_CHUNK 4096
+#define L2O_APPLY_THREADS 256
+#define L2O_ILS_THREADS 1024

@@ -1,0 +1,340 @@
+"""Python binding (ctypes) of the C ABI in include/logo2opt.h.
+
+Thin host-side mirror for tests and the benchmark: every call goes straight to
+liblogo2opt.so (HIP kernels on the GPU).  There is no CPU fallback: if the
+library is missing or a call fails, an exception is raised.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblogo2opt.so")
+
+KICK_DOUBLE_BRIDGE = 0
+KICK_REFERENCE = 1
+MAX_N = 1 << 20
+
+# every entry point the header declares (checked by the CPU test suite)
+EXPORTS = [
+    "logo2opt_strerror", "logo2opt_create", "logo2opt_destroy", "logo2opt_step", "logo2opt_optimize",
+    "logo2opt_ils", "logo2opt_route_length", "logo2opt_get_stats", "logo2opt_set_timing", "logo2opt_time_sweep",
+    "logo2opt_engine_create", "logo2opt_engine_destroy", "logo2opt_engine_bytes_for", "logo2opt_engine_add",
+    "logo2opt_engine_run", "logo2opt_engine_result", "logo2opt_engine_round_lengths", "logo2opt_engine_remove",
+    "logo2opt_engine_poll_done", "logo2opt_engine_stats", "logo2opt_engine_set_timing", "logo2opt_gen_uniform", "logo2opt_gen_clustered",
+    "logo2opt_random_perm", "logo2opt_nn_tour", "logo2opt_xy_to_coords", "logo2opt_sqrt_ulp_check",
+]
+
+
+class CityCoords(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class Best2Out(C.Structure):
+    _fields_ = [("i", C.c_uint32), ("j", C.c_uint32), ("minchange", C.c_int32)]
+
+
+class ProcessTime(C.Structure):
+    _fields_ = [("kernel_time", C.c_double), ("HtD_time", C.c_double), ("DtH_time", C.c_double),
+                ("out", Best2Out)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("sweeps", C.c_uint64), ("moves", C.c_uint64), ("evals", C.c_uint64),
+                ("fallbacks", C.c_uint64), ("sweep_ms", C.c_double), ("wall_ms", C.c_double),
+                ("sweep_launches", C.c_uint64)]
+
+
+class Job(C.Structure):
+    _fields_ = [("ils_rounds", C.c_uint32), ("kick", C.c_uint32), ("seed", C.c_uint64),
+                ("log_cap", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("length", C.c_uint64), ("moves", C.c_uint64), ("sweeps", C.c_uint64),
+                ("fallbacks", C.c_uint64), ("rounds_done", C.c_uint32), ("done", C.c_uint32)]
+
+
+class Logo2OptError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {strerror(code)} ({code})")
+        self.code = code
+
+
+_lib = None
+vp = C.c_void_p
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def load():
+    """Loads liblogo2opt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with logo-tsp-solver_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    L.logo2opt_strerror.argtypes = [C.c_int]
+    L.logo2opt_strerror.restype = C.c_char_p
+    L.logo2opt_create.argtypes = [C.c_int, C.c_uint32, vp, C.POINTER(vp)]
+    L.logo2opt_destroy.argtypes = [vp]
+    L.logo2opt_step.argtypes = [vp, u32p, C.POINTER(Best2Out), C.POINTER(ProcessTime)]
+    L.logo2opt_optimize.argtypes = [vp, u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp, C.c_uint64,
+                                    C.c_uint32]
+    L.logo2opt_ils.argtypes = [vp, u32p, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), vp]
+    L.logo2opt_route_length.argtypes = [vp, u32p, C.POINTER(C.c_uint64)]
+    L.logo2opt_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.logo2opt_set_timing.argtypes = [vp, C.c_int]
+    L.logo2opt_engine_set_timing.argtypes = [vp, C.c_int]
+    L.logo2opt_time_sweep.argtypes = [vp, u32p, C.c_uint32, C.POINTER(C.c_double)]
+    L.logo2opt_engine_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(vp)]
+    L.logo2opt_engine_destroy.argtypes = [vp]
+    L.logo2opt_engine_bytes_for.argtypes = [C.c_uint32, C.POINTER(Job)]
+    L.logo2opt_engine_bytes_for.restype = C.c_uint64
+    L.logo2opt_engine_add.argtypes = [vp, C.c_uint32, vp, u32p, C.POINTER(Job), C.POINTER(C.c_int32)]
+    L.logo2opt_engine_run.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_engine_result.argtypes = [vp, C.c_int32, vp, C.POINTER(Result), vp]
+    L.logo2opt_engine_round_lengths.argtypes = [vp, C.c_int32, u64p, C.c_uint32]
+    L.logo2opt_engine_remove.argtypes = [vp, C.c_int32]
+    L.logo2opt_engine_poll_done.argtypes = [vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_engine_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.logo2opt_gen_uniform.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, i32p]
+    L.logo2opt_gen_uniform.restype = None
+    L.logo2opt_gen_clustered.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                         C.c_uint64, i32p]
+    L.logo2opt_gen_clustered.restype = None
+    L.logo2opt_random_perm.argtypes = [C.c_uint32, C.c_uint64, u32p]
+    L.logo2opt_random_perm.restype = None
+    L.logo2opt_nn_tour.argtypes = [C.c_uint32, i32p, C.c_uint32, u32p]
+    L.logo2opt_xy_to_coords.argtypes = [C.c_uint32, i32p, vp]
+    L.logo2opt_xy_to_coords.restype = None
+    L.logo2opt_sqrt_ulp_check.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
+    _lib = L
+    return L
+
+
+def strerror(code):
+    return load().logo2opt_strerror(code).decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise Logo2OptError(rc, what)
+
+
+# ---- workloads -------------------------------------------------------------
+def gen_uniform(n, side, seed):
+    xy = np.zeros(2 * n, np.int32)
+    load().logo2opt_gen_uniform(n, side, seed, xy)
+    return xy
+
+
+def gen_clustered(n, side, seed, nclusters=60, sigma_frac=0.02, noise_frac=0.10):
+    xy = np.zeros(2 * n, np.int32)
+    load().logo2opt_gen_clustered(n, side, nclusters, sigma_frac, noise_frac, seed, xy)
+    return xy
+
+
+def random_perm(n, seed):
+    r = np.zeros(n, np.uint32)
+    load().logo2opt_random_perm(n, seed, r)
+    return r
+
+
+def nn_tour(xy, start=0):
+    xy = np.ascontiguousarray(xy, np.int32)
+    r = np.zeros(len(xy) // 2, np.uint32)
+    _check(load().logo2opt_nn_tour(len(r), xy, start, r), "nn_tour")
+    return r
+
+
+def coords_of(xy):
+    """int32 interleaved xy -> array of reference city_coords (float x, y)."""
+    xy = np.ascontiguousarray(xy, np.int32)
+    n = len(xy) // 2
+    c = (CityCoords * n)()
+    load().logo2opt_xy_to_coords(n, xy, C.cast(c, vp))
+    return c
+
+
+def float_coords(x, y):
+    n = len(x)
+    c = (CityCoords * n)()
+    arr = np.frombuffer(c, dtype=np.float32).reshape(n, 2)
+    arr[:, 0] = x
+    arr[:, 1] = y
+    return c
+
+
+def sqrt_ulp_check(lo_bits, hi_bits, device=0):
+    """Max ulp distance of v_sqrt_f32 from the correctly rounded sqrt over the
+    float bit patterns [lo_bits, hi_bits), measured on the GPU."""
+    m = C.c_uint32()
+    _check(load().logo2opt_sqrt_ulp_check(device, lo_bits, hi_bits, C.byref(m)), "sqrt_ulp_check")
+    return m.value
+
+
+# ---- single-instance context (the CUDASolver replacement) -----------------
+class Context:
+    def __init__(self, coords, n=None, device=0):
+        L = load()
+        if n is None:
+            n = len(coords)
+        self.n = n
+        self._coords = coords  # keep alive
+        h = vp()
+        _check(L.logo2opt_create(device, n, C.cast(coords, vp), C.byref(h)), "logo2opt_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().logo2opt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, route, timing=False):
+        route = np.ascontiguousarray(route, np.uint32)
+        o = Best2Out()
+        t = ProcessTime()
+        _check(load().logo2opt_step(self.h, route, C.byref(o), C.byref(t) if timing else None), "logo2opt_step")
+        out = (o.i, o.j, o.minchange)
+        return (out, (t.HtD_time, t.kernel_time, t.DtH_time)) if timing else out
+
+    def optimize(self, route, log_cap=0, sync_every=64):
+        r = np.array(route, np.uint32)
+        L = C.c_uint64()
+        m = C.c_uint64()
+        log = (Best2Out * max(1, log_cap))() if log_cap else None
+        _check(load().logo2opt_optimize(self.h, r, C.byref(L), C.byref(m), C.cast(log, vp) if log else None,
+                                        log_cap, sync_every), "logo2opt_optimize")
+        out = dict(route=r, length=int(L.value), moves=int(m.value))
+        if log_cap:
+            k = min(out["moves"], log_cap)
+            out["log"] = np.array([(log[q].i, log[q].j, log[q].minchange) for q in range(k)], np.int64).reshape(k, 3)
+        return out
+
+    def ils(self, route, seed, rounds, kick=KICK_DOUBLE_BRIDGE):
+        r = np.array(route, np.uint32)
+        best = C.c_uint64()
+        rl = np.zeros(rounds + 1, np.uint64)
+        _check(load().logo2opt_ils(self.h, r, seed, rounds, kick, C.byref(best), rl.ctypes.data),
+               "logo2opt_ils")
+        return r, int(best.value), rl
+
+    def route_length(self, route):
+        L = C.c_uint64()
+        _check(load().logo2opt_route_length(self.h, np.ascontiguousarray(route, np.uint32), C.byref(L)),
+               "logo2opt_route_length")
+        return int(L.value)
+
+    def stats(self):
+        s = Stats()
+        _check(load().logo2opt_get_stats(self.h, C.byref(s)), "logo2opt_get_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def set_timing(self, on):
+        _check(load().logo2opt_set_timing(self.h, 1 if on else 0), "logo2opt_set_timing")
+
+    def time_sweep(self, route, sweeps):
+        ms = C.c_double()
+        _check(load().logo2opt_time_sweep(self.h, np.ascontiguousarray(route, np.uint32), sweeps, C.byref(ms)),
+               "logo2opt_time_sweep")
+        return ms.value
+
+
+# ---- engine (batched instances, ILS, serving) ------------------------------
+class Engine:
+    def __init__(self, device=0, hbm_budget=1 << 30, max_slots=4096):
+        h = vp()
+        _check(load().logo2opt_engine_create(device, hbm_budget, max_slots, C.byref(h)), "logo2opt_engine_create")
+        self.h = h
+        self.max_slots = max_slots
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            load().logo2opt_engine_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def bytes_for(n, ils_rounds=0, log_cap=0):
+        j = Job(ils_rounds, 0, 0, log_cap, 0)
+        return int(load().logo2opt_engine_bytes_for(n, C.byref(j)))
+
+    def add(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, log_cap=0):
+        """Returns the slot id, or None when the engine is full (EAGAIN)."""
+        route = np.ascontiguousarray(route, np.uint32)
+        j = Job(ils_rounds, kick, seed, log_cap, 0)
+        s = C.c_int32()
+        rc = load().logo2opt_engine_add(self.h, len(route), C.cast(coords, vp), route, C.byref(j), C.byref(s))
+        if rc == -6:
+            return None
+        _check(rc, "logo2opt_engine_add")
+        return s.value
+
+    def run(self, max_steps=0xFFFFFFFF, sync_every=32):
+        live = C.c_uint32()
+        _check(load().logo2opt_engine_run(self.h, max_steps, sync_every, C.byref(live)), "logo2opt_engine_run")
+        return live.value
+
+    def result(self, slot, n=None, log_cap=0):
+        res = Result()
+        r = np.zeros(n, np.uint32) if n else None
+        log = (Best2Out * max(1, log_cap))() if log_cap else None
+        _check(load().logo2opt_engine_result(self.h, slot, r.ctypes.data if n else None, C.byref(res),
+                                             C.cast(log, vp) if log else None), "logo2opt_engine_result")
+        out = {f: getattr(res, f) for f, _ in Result._fields_}
+        if n:
+            out["route"] = r
+        if log_cap:
+            k = min(out["moves"], log_cap)
+            out["log"] = np.array([(log[q].i, log[q].j, log[q].minchange) for q in range(k)], np.int64).reshape(k, 3)
+        return out
+
+    def round_lengths(self, slot, cap):
+        out = np.zeros(cap, np.uint64)
+        _check(load().logo2opt_engine_round_lengths(self.h, slot, out, cap), "round_lengths")
+        return out
+
+    def remove(self, slot):
+        _check(load().logo2opt_engine_remove(self.h, slot), "logo2opt_engine_remove")
+
+    def poll_done(self, cap=4096):
+        buf = (C.c_int32 * cap)()
+        cnt = C.c_uint32()
+        _check(load().logo2opt_engine_poll_done(self.h, C.cast(buf, vp), cap, C.byref(cnt)), "poll_done")
+        return [buf[k] for k in range(cnt.value)]
+
+    def set_timing(self, on):
+        _check(load().logo2opt_engine_set_timing(self.h, 1 if on else 0), "logo2opt_engine_set_timing")
+
+    def stats(self):
+        s = Stats()
+        _check(load().logo2opt_engine_stats(self.h, C.byref(s)), "logo2opt_engine_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}

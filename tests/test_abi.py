@@ -1,0 +1,93 @@
+"""CPU-side checks of the product library: it loads, exports every entry point
+include/logo2opt.h declares, its host-side workload code matches the oracle
+bit for bit, and argument validation fails with the documented error codes
+before any device work (no GPU needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import logo2opt as L
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "logo2opt.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(logo2opt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (logo2opt_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = L.load()
+    for n in names:
+        assert hasattr(lib, n)
+    assert sorted(L.EXPORTS) == names
+
+
+def test_struct_layouts_match_reference():
+    # city_coords 8 B, best2_out 12 B (defs.h:173-187); process_time = 3 doubles + best2_out
+    assert C.sizeof(L.CityCoords) == 8
+    assert C.sizeof(L.Best2Out) == 12
+    assert C.sizeof(L.ProcessTime) == 3 * 8 + 12 + 4  # padded to 8
+    assert C.sizeof(L.Job) == 24 and C.sizeof(L.Result) == 40
+
+
+def test_strerror_codes():
+    for code in range(0, -9, -1):
+        assert L.strerror(code)
+    assert L.strerror(-100) == "unknown error"
+
+
+@pytest.mark.parametrize("n,side,seed", [(1000, 10**4, 1), (33810, 10**6, 2), (5000, 10**4, 3)])
+def test_uniform_matches_oracle(n, side, seed):
+    assert (L.gen_uniform(n, side, seed) == O.gen_uniform(n, side, seed)).all()
+
+
+def test_clustered_matches_oracle():
+    assert (L.gen_clustered(33810, 10**6, 7) == O.gen_clustered(33810, 10**6, 7)).all()
+    assert (L.gen_clustered(5000, 10**4, 8, 10, 0.05, 0.2) == O.gen_clustered(5000, 10**4, 8, 10, 0.05, 0.2)).all()
+
+
+def test_random_perm_matches_oracle():
+    for n, s in [(4, 1), (1000, 2), (100000, 3)]:
+        assert (L.random_perm(n, s) == O.random_perm(n, s)).all()
+
+
+@pytest.mark.parametrize("kind,n,side", [("uniform", 3000, 10**4), ("uniform", 2000, 60),
+                                         ("clustered", 8000, 10**6), ("uniform", 500, 3)])
+def test_nn_tour_matches_oracle(kind, n, side):
+    xy = L.gen_uniform(n, side, 11) if kind == "uniform" else L.gen_clustered(n, side, 12)
+    for start in (0, n // 2):
+        assert (L.nn_tour(xy, start) == O.nn_tour(xy, start)).all()
+
+
+def test_invalid_inputs_rejected_before_device_work():
+    lib = L.load()
+    h = C.c_void_p()
+    # non-integral coordinates
+    c = L.float_coords(np.array([0, 1.5, 2, 3], np.float32), np.zeros(4, np.float32))
+    assert lib.logo2opt_create(0, 4, C.cast(c, C.c_void_p), C.byref(h)) == -2
+    # span >= 2^22
+    c = L.float_coords(np.array([0, 1 << 22, 2, 3], np.float32), np.zeros(4, np.float32))
+    assert lib.logo2opt_create(0, 4, C.cast(c, C.c_void_p), C.byref(h)) == -2
+    # n out of range
+    c = L.float_coords(np.array([0, 1], np.float32), np.zeros(2, np.float32))
+    assert lib.logo2opt_create(0, 2, C.cast(c, C.c_void_p), C.byref(h)) == -1
+    assert lib.logo2opt_create(0, 4, None, C.byref(h)) == -1
+
+
+def test_engine_bytes_for():
+    assert L.Engine.bytes_for(1000) == 2 * 16128 + 2 * 4096
+    assert L.Engine.bytes_for(1000, ils_rounds=10) > L.Engine.bytes_for(1000)
