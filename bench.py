@@ -133,6 +133,59 @@ def batched_vs_plain(count, plain_every, seed=4):
     }
 
 
+def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=512):
+    """Open-loop Poisson replay through the continuous-batching server."""
+    rs = np.random.default_rng(seed)
+    insts = []
+    for k in range(pool):
+        n = int(np.rint(np.exp(rs.uniform(np.log(200), np.log(50000)))))
+        xy = L.gen_clustered(n, 10**6, 5000 + k) if rs.random() < 0.5 else L.gen_uniform(n, 10**6, 5000 + k)
+        insts.append((n, L.coords_of(xy), L.nn_tour(xy, 0)))
+    out = []
+    for lam in rates:
+        gaps = rs.exponential(1.0 / lam, int(lam * window_s * 1.5) + 16)
+        arrivals = np.cumsum(gaps)
+        arrivals = arrivals[arrivals <= window_s]
+        pick = rs.integers(0, pool, len(arrivals))
+        reqs = [(insts[p][1], insts[p][2], L.Job(rounds, 0, 7000 + q, 0, 0)) for q, p in enumerate(pick)]
+        with L.Server(hbm_budget=8 << 30, max_slots=max_slots, queue_cap=1 << 20) as srv:
+            lat, qs, el = srv.replay(arrivals, reqs, horizon_s, steps=32, sync_every=16)
+            cnt = srv.counts()
+        ok = lat >= 0
+        done_n = [insts[p][0] for p in pick[ok]]
+        out.append({
+            "rate_req_per_s": lam, "arrived": int(len(arrivals)), "completed": int(ok.sum()),
+            "throughput_req_per_s": float(ok.sum() / el), "elapsed_s": el,
+            "p50_latency_s": float(np.percentile(lat[ok], 50)) if ok.any() else None,
+            "p99_latency_s": float(np.percentile(lat[ok], 99)) if ok.any() else None,
+            "p50_queue_s": float(np.percentile(qs[ok], 50)) if ok.any() else None,
+            "mean_n_completed": float(np.mean(done_n)) if done_n else None,
+            "admitted": int(cnt["admitted"]), "rejected": int(cnt["rejected"]),
+            "left_running": int(cnt["running"]), "left_queued": int(cnt["queued"]),
+        })
+        log(f"serving lambda={lam}: {out[-1]}")
+    return {"workload": f"ILS requests ({rounds} double-bridge rounds, greedy NN start), n log-uniform in "
+                        f"[200,50000], uniform or clustered coords in [0,1e6)^2 (pool of {pool} instances, "
+                        f"distinct kick seeds), Poisson arrivals for {window_s}s, horizon {horizon_s}s",
+            "max_slots": max_slots, "rates": out}
+
+
+def config3(rounds, start, seed=3):
+    """ILS on 100,000 uniform cities: LO, then `rounds` double-bridge kicks."""
+    n = 100000
+    xy = L.gen_uniform(n, 10**6, seed)
+    r0 = L.nn_tour(xy, 0) if start == "nn" else L.random_perm(n, seed + 1)
+    with L.Context(L.coords_of(xy)) as ctx:
+        t0 = time.perf_counter()
+        r, best, rl = ctx.ils(r0, seed=seed + 2, rounds=rounds)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+    return {"workload": f"ILS on 100,000 uniform cities in [0,1e6)^2, {start} start, {rounds} seeded "
+                        "double-bridge rounds, strict accept", "seconds": dt, "best_length": best,
+            "length_after_first_lo": int(rl[0]), "moves": st["moves"], "sweeps": st["sweeps"],
+            "evals_per_s": st["evals"] / dt, "fallbacks": st["fallbacks"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,7 +196,17 @@ def main():
     ap.add_argument("--batch-count", type=int, default=2048)
     ap.add_argument("--plain-every", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="50")
+    ap.add_argument("--serve-window", type=float, default=4.0)
+    ap.add_argument("--serve-horizon", type=float, default=12.0)
+    ap.add_argument("--config3", action="store_true", help="also run the 100k-city ILS (long)")
+    ap.add_argument("--config3-rounds", type=int, default=500)
+    ap.add_argument("--config3-start", default="nn", choices=["nn", "random"])
+    ap.add_argument("--no-graph", action="store_true", help="direct launches instead of HIP graph replay")
     args = ap.parse_args()
+    if args.no_graph:
+        os.environ["LOGO2OPT_GRAPHS"] = "0"
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
@@ -204,6 +267,13 @@ def main():
     if not args.skip_batch:
         log("batched vs plain")
         res["batched"] = batched_vs_plain(args.batch_count, args.plain_every)
+    if not args.skip_serve:
+        log("serving")
+        rates = [float(v) for v in args.serve_rates.split(",")]
+        res["serving"] = serving(rates, args.serve_window, args.serve_horizon)
+    if args.config3:
+        log("config3 ILS 100k")
+        res["ils_100k"] = config3(args.config3_rounds, args.config3_start)
     print(json.dumps(res), flush=True)
 
 

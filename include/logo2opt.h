@@ -186,6 +186,64 @@ int logo2opt_engine_remove(logo2opt_engine *e, int32_t slot);
 int logo2opt_engine_poll_done(logo2opt_engine *e, int32_t *out, uint32_t cap, uint32_t *count);
 int logo2opt_engine_stats(logo2opt_engine *e, logo2opt_stats *st);
 int logo2opt_engine_set_timing(logo2opt_engine *e, int on);
+/* on = 0: issue every step as direct launches instead of replaying captured
+ * HIP graphs (also LOGO2OPT_GRAPHS=0 in the environment at engine creation). */
+int logo2opt_engine_set_graphs(logo2opt_engine *e, int on);
+
+/* ======================================================================== */
+/* Server: continuous batching of local-search / ILS requests                */
+/* ======================================================================== */
+/* No counterpart in the reference (its multi-start driver runs one solver per
+ * pthread, ILSGlobalSolverMT.cpp:86-195).  Requests queue FIFO on the host;
+ * each pump admits them into engine slots while the HBM budget and slot count
+ * allow, advances every admitted request by a batch of steps, and harvests
+ * the finished ones, freeing their HBM for the next admissions. */
+typedef struct logo2opt_server logo2opt_server;
+int logo2opt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots, uint32_t queue_cap,
+                           logo2opt_server **out);
+int logo2opt_server_destroy(logo2opt_server *s);
+/* Queue a request (copied).  LOGO2OPT_EAGAIN (id = -1) when the queue is full. */
+int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_coords *coords,
+                           const uint32_t *route, const logo2opt_job *job, int64_t *id);
+/* Admit what fits, run up to `steps` batched steps (host check every
+ * sync_every), harvest finished requests. */
+int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every, uint32_t *running,
+                         uint32_t *queued);
+/* Ids completed since the last poll. */
+int logo2opt_server_poll(logo2opt_server *s, int64_t *ids, uint32_t cap, uint32_t *count);
+/* Result of a completed request; latency_s = submit -> done, queue_s = submit -> admission. */
+int logo2opt_server_result(logo2opt_server *s, int64_t id, uint32_t *route_out, logo2opt_result *res,
+                           double *latency_s, double *queue_s);
+/* Forget a request (cancels it if queued or running). */
+int logo2opt_server_release(logo2opt_server *s, int64_t id);
+/* counts[6] = {submitted, rejected, admitted, done, queued now, running now} */
+int logo2opt_server_counts(logo2opt_server *s, uint64_t *counts);
+/* Open-loop replay of `count` requests arriving at arrival_s[q] seconds
+ * (ascending) after the start; runs until all finished or horizon_s.
+ * latency_out[q]: arrival -> completion (-1 if rejected or unfinished);
+ * queue_out[q] (optional): arrival -> admission. */
+int logo2opt_server_replay(logo2opt_server *s, uint32_t count, const double *arrival_s, const uint32_t *n,
+                           const logo2opt_city_coords *const *coords, const uint32_t *const *routes,
+                           const logo2opt_job *jobs, double horizon_s, uint32_t steps, uint32_t sync_every,
+                           double *latency_out, double *queue_out, double *elapsed_s);
+
+/* ======================================================================== */
+/* Sweep tile shapes (autotuning)                                            */
+/* ======================================================================== */
+/* A shape is {k, t, waves}: a workgroup of `waves` wave64s whose lanes each
+ * walk k adjacent diagonals of the candidate triangle for t steps.  The
+ * reference hard-codes <<<64,1024>>> and 2048-point tiles
+ * (CUDASolver.cpp:122-128, cuda_defs.h:24-37); here the shape is chosen per
+ * n bucket from a table written by the autotuner (tune.py). */
+/* Shapes compiled into the library (3 ints per row); count = total rows. */
+int logo2opt_tuning_shapes(int32_t *out, uint32_t cap, uint32_t *count);
+/* Replace the table: row r applies to n <= max_n[r] (the last row to all
+ * larger n).  Affects instances admitted afterwards. */
+int logo2opt_tuning_set(const uint32_t *max_n, const int32_t *shapes, uint32_t rows);
+int logo2opt_tuning_get(uint32_t *max_n, int32_t *shapes, uint32_t cap, uint32_t *rows);
+/* Load a table file ("max_n k t waves" per line).  The file named by the
+ * LOGO2OPT_TUNING environment variable is loaded on first use. */
+int logo2opt_tuning_load(const char *path);
 
 /* ======================================================================== */
 /* Synthetic workloads and construction (host, deterministic)               */

@@ -12,6 +12,8 @@
 // CUDASolver.cpp:48-434) is an engine with one slot.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -60,28 +62,53 @@ std::vector<tune_row> g_tune = {
 };
 bool g_tune_loaded = false;
 
-void load_tuning_locked()
+// text table, one "max_n k t waves" row per line; returns rows accepted
+int read_tuning_file(const char *path, std::vector<tune_row> &rows)
 {
-    if (g_tune_loaded) return;
-    g_tune_loaded = true;
-    const char *p = std::getenv("LOGO2OPT_TUNING");
-    std::string path = p ? p : "";
-    if (path.empty()) return;
-    FILE *f = std::fopen(path.c_str(), "r");
-    if (!f) return;
-    std::vector<tune_row> rows;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return -1;
     unsigned mx;
     int k, t, w;
     while (std::fscanf(f, "%u %d %d %d", &mx, &k, &t, &w) == 4) {
         sweep_cfg c{k, t, w};
-        if (l2o::sweep_cfg_supported(c)) rows.push_back({mx, c});
+        if (!l2o::sweep_cfg_supported(c)) {
+            std::fclose(f);
+            return -1;
+        }
+        rows.push_back({mx, c});
     }
     std::fclose(f);
-    if (!rows.empty()) {
-        std::sort(rows.begin(), rows.end(), [](const tune_row &a, const tune_row &b) { return a.max_n < b.max_n; });
-        rows.back().max_n = LOGO2OPT_MAX_N;
-        g_tune = rows;
+    return (int)rows.size();
+}
+
+void install_rows_locked(std::vector<tune_row> rows)
+{
+    std::sort(rows.begin(), rows.end(), [](const tune_row &a, const tune_row &b) { return a.max_n < b.max_n; });
+    rows.back().max_n = LOGO2OPT_MAX_N;
+    g_tune = rows;
+}
+
+// $LOGO2OPT_TUNING, else tuning_gfx950.txt next to this library (written by
+// tune.py), else the built-in defaults above
+void load_tuning_locked()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    std::string path;
+    const char *p = std::getenv("LOGO2OPT_TUNING");
+    if (p && *p) {
+        path = p;
+    } else {
+        Dl_info info;
+        if (dladdr((void *)&load_tuning_locked, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string() : path.substr(0, slash + 1)) + "tuning_gfx950.txt";
+        }
     }
+    if (path.empty()) return;
+    std::vector<tune_row> rows;
+    if (read_tuning_file(path.c_str(), rows) > 0) install_rows_locked(rows);
 }
 
 sweep_cfg cfg_for(uint32_t n)
@@ -219,6 +246,9 @@ struct logo2opt_engine {
     // timing mode: every step's sweep launches are bracketed by HIP events on
     // the engine stream (no graphs); elapsed times are summed at each sync
     bool timing = false;
+    // batches of steps are captured into a HIP graph unless disabled
+    // (logo2opt_engine_set_graphs, or LOGO2OPT_GRAPHS=0 in the environment)
+    bool graphs = true;
     std::vector<hipEvent_t> tev;
     uint32_t tev_used = 0;
 };
@@ -458,7 +488,7 @@ int collect_timing(logo2opt_engine *e)
 
 int issue_steps(logo2opt_engine *e, uint32_t k, uint32_t sync_every)
 {
-    if (sync_every > 1 && k == sync_every && !e->timing) {
+    if (sync_every > 1 && k == sync_every && !e->timing && e->graphs) {
         if (!e->gexec || e->gsteps != k) {
             drop_graph(e);
             hipGraph_t g;
@@ -649,6 +679,7 @@ int engine_create(int device, uint64_t budget, uint32_t max_slots, logo2opt_engi
     e->device = device;
     e->max_slots = max_slots;
     e->slots.resize(max_slots);
+    if (const char *g = std::getenv("LOGO2OPT_GRAPHS")) e->graphs = std::strcmp(g, "0") != 0;
     bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&e->d_arena, budget) == hipSuccess &&
               hipMalloc(&e->d_insts, sizeof(l2o_inst) * max_slots) == hipSuccess &&
@@ -697,6 +728,60 @@ const char *logo2opt_strerror(int code)
 int logo2opt_engine_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots, logo2opt_engine **out)
 {
     return engine_create(device, hbm_budget_bytes, max_slots, out);
+}
+
+int logo2opt_tuning_shapes(int32_t *out, uint32_t cap, uint32_t *count)
+{
+    if (!count) return LOGO2OPT_EINVAL;
+    int tmp[64][3];
+    const int m = l2o::sweep_shapes(tmp, 64);
+    for (int q = 0; q < m && (uint32_t)q < cap && out; ++q)
+        for (int c = 0; c < 3; ++c) out[3 * q + c] = tmp[q][c];
+    *count = (uint32_t)m;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_tuning_set(const uint32_t *max_n, const int32_t *shapes, uint32_t rows)
+{
+    if (!max_n || !shapes || rows == 0) return LOGO2OPT_EINVAL;
+    std::vector<tune_row> v;
+    for (uint32_t r = 0; r < rows; ++r) {
+        sweep_cfg c{shapes[3 * r], shapes[3 * r + 1], shapes[3 * r + 2]};
+        if (!l2o::sweep_cfg_supported(c)) return LOGO2OPT_EINVAL;
+        v.push_back({max_n[r], c});
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_loaded = true;
+    install_rows_locked(v);
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_tuning_get(uint32_t *max_n, int32_t *shapes, uint32_t cap, uint32_t *rows)
+{
+    if (!rows) return LOGO2OPT_EINVAL;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tuning_locked();
+    for (uint32_t r = 0; r < g_tune.size() && r < cap; ++r) {
+        if (max_n) max_n[r] = g_tune[r].max_n;
+        if (shapes) {
+            shapes[3 * r] = g_tune[r].cfg.k;
+            shapes[3 * r + 1] = g_tune[r].cfg.t;
+            shapes[3 * r + 2] = g_tune[r].cfg.waves;
+        }
+    }
+    *rows = (uint32_t)g_tune.size();
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_tuning_load(const char *path)
+{
+    if (!path) return LOGO2OPT_EINVAL;
+    std::vector<tune_row> v;
+    if (read_tuning_file(path, v) <= 0) return LOGO2OPT_EINVAL;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_loaded = true;
+    install_rows_locked(v);
+    return LOGO2OPT_OK;
 }
 
 int logo2opt_engine_destroy(logo2opt_engine *e)
@@ -800,6 +885,14 @@ int logo2opt_set_timing(logo2opt_ctx *ctx, int on)
 {
     if (!ctx) return LOGO2OPT_EINVAL;
     return logo2opt_engine_set_timing(ctx->e, on);
+}
+
+int logo2opt_engine_set_graphs(logo2opt_engine *e, int on)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    e->graphs = on != 0;
+    drop_graph(e);
+    return LOGO2OPT_OK;
 }
 
 // ---------------------------------------------------------------------------

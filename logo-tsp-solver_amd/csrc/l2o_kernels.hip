@@ -39,8 +39,9 @@ namespace l2o {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int exact_dist(float x1, float y1, float x2, float y2)
 {
-    const long long dx = (long long)x1 - (long long)x2;
-    const long long dy = (long long)y1 - (long long)y2;
+    // coordinates are integers below 2^22: int32 conversion is exact
+    const long long dx = (long long)((int)x1 - (int)x2);
+    const long long dy = (long long)((int)y1 - (int)y2);
     const long long s = dx * dx + dy * dy;
     long long k = (long long)sqrt((double)s);
     if (k * k > s) k -= 1;
@@ -81,6 +82,22 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 // ---------------------------------------------------------------------------
 // sweep
 // ---------------------------------------------------------------------------
+// LDS layout of one workgroup (all float):
+//   rx, ry, rd [T+1]   rows, index 0 = P[a0], index t+1 = P[a0+t+1] (SoA so
+//                      that four consecutive steps are one broadcast read)
+//   col[T + DIAG]      float4 {x, y, D, 0} of P[a0+g0+c]
+//   u[WAVES][T]        per-wave thr + D[a0+t+1]: the whole per-step
+//                      threshold comes from LDS, so a candidate costs
+//                      2 sub + mul + fma + sqrt + add + sub + cmp on the VALU
+template <int K, int T, int WAVES>
+struct sweep_smem {
+    float rx[T + 4], ry[T + 4], rd[T + 4];
+    float4 col[T + 64 * WAVES * K + 4];
+    float u[WAVES][T];
+    unsigned long long key[WAVES];
+    unsigned fb[WAVES];
+};
+
 template <int K, int T, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__restrict__ items,
                                                       l2o_inst *__restrict__ insts)
@@ -88,10 +105,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
     constexpr int NT = 64 * WAVES;
     constexpr int DIAG = NT * K;
     constexpr int NCOL = T + DIAG;
-    __shared__ float4 s_rows[T + 1];
-    __shared__ float4 s_cols[NCOL];
-    __shared__ unsigned long long s_key[WAVES];
-    __shared__ unsigned s_fb[WAVES];
+    constexpr int UF = 4;  // steps per unrolled iteration
+    static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
+    __shared__ sweep_smem<K, T, WAVES> sm;
 
     const l2o_sweep_item it = items[blockIdx.x];
     l2o_inst *in = insts + it.inst;
@@ -99,108 +115,144 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
     const int n = (int)in->n;
     const float4 *__restrict__ P = in->pts[in->parity];
     const int a0 = (int)it.a0, g0 = (int)it.g0;
+    const float slack = in->slack;
 
-    for (int c = threadIdx.x; c < NCOL; c += NT) s_cols[c] = P[min(a0 + g0 + c, n - 1)];
-    for (int r = threadIdx.x; r <= T; r += NT) s_rows[r] = P[min(a0 + r, n - 1)];
+    for (int c = threadIdx.x; c < NCOL; c += NT) sm.col[c] = P[min(a0 + g0 + c, n - 1)];
+    for (int r = threadIdx.x; r <= T; r += NT) {
+        const float4 p = P[min(a0 + r, n - 1)];
+        sm.rx[r] = p.x;
+        sm.ry[r] = p.y;
+        sm.rd[r] = p.z;
+    }
     __syncthreads();
 
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int cb = w * 64 * K + lane * K;  // this lane's first diagonal, relative to g0
     const int gmin = g0 + w * 64 * K, gmax = gmin + 64 * K - 1;
     // candidate (a, a+g) exists iff a + g <= n-2, i.e. step t <= n-2-g-a0
-    const int t_full = max(0, min(T, n - 1 - gmax - a0));  // all lanes valid below
+    const int t_full = max(0, min(T, n - 1 - gmax - a0));  // every lane valid below
     const int t_end = max(0, min(T, n - 1 - gmin - a0));   // no lane valid from here
-    const float slack = in->slack;
+    float *__restrict__ su = sm.u[w];
 
-    float qprev[K];
+    float thr = slack - 1.0f;  // (best_gain - 1) + slack with best_gain = 0: only gain <= -1 matters
+    for (int t = lane; t < T; t += 64) su[t] = thr + sm.rd[t + 1];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    float qp[K];
     {
-        const float4 R0 = s_rows[0];
+        const float x0 = sm.rx[0], y0 = sm.ry[0];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float4 C = s_cols[cb + k];
-            const float dx = R0.x - C.x, dy = R0.y - C.y;
-            qprev[k] = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+            const float4 C = sm.col[cb + k];
+            const float dx = x0 - C.x, dy = y0 - C.y;
+            qp[k] = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
         }
     }
     unsigned long long lbest = L2O_SENTINEL;
-    float thr = slack - 1.0f;  // (best_gain - 1) + slack with best_gain = 0: only gain <= -1 matters
     unsigned fb = 0;
 
-    // exact re-evaluation of the flagged candidates of step t; updates thr
-    auto slow = [&](int t, const bool *hit) {
-        const float4 Ra = s_rows[t], Rb = s_rows[t + 1];
+    // exact re-evaluation of flagged candidates (steps t0 .. t0+ns-1); then
+    // the wave's threshold and its per-step thresholds in LDS are refreshed
+    auto slow = [&](int t0, int ns, const bool *hit) {
+        for (int s = 0; s < ns; ++s) {
+            const int t = t0 + s;
+            const float xa = sm.rx[t], ya = sm.ry[t], xb = sm.rx[t + 1], yb = sm.ry[t + 1], db = sm.rd[t + 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (hit[k]) {
-                ++fb;
-                const float4 Ca = s_cols[t + cb + k], Cb = s_cols[t + 1 + cb + k];
-                const int gain = exact_dist(Ra.x, Ra.y, Ca.x, Ca.y) + exact_dist(Rb.x, Rb.y, Cb.x, Cb.y) -
-                                 (int)Rb.z - (int)Cb.z;
-                if (gain < 0) {
-                    const uint32_t i = (uint32_t)(a0 + t + 1);
-                    const uint32_t j = (uint32_t)(a0 + t + 1 + g0 + cb + k);
-                    const unsigned long long key = make_key(gain, i, j);
-                    lbest = key < lbest ? key : lbest;
+            for (int k = 0; k < K; ++k) {
+                if (hit[s * K + k]) {
+                    ++fb;
+                    const float4 Ca = sm.col[t + cb + k], Cb = sm.col[t + 1 + cb + k];
+                    const int gain = exact_dist(xa, ya, Ca.x, Ca.y) + exact_dist(xb, yb, Cb.x, Cb.y) - (int)db -
+                                     (int)Cb.z;
+                    if (gain < 0) {
+                        const uint32_t i = (uint32_t)(a0 + t + 1);
+                        const uint32_t j = (uint32_t)(a0 + t + 1 + g0 + cb + k);
+                        const unsigned long long key = make_key(gain, i, j);
+                        lbest = key < lbest ? key : lbest;
+                    }
                 }
             }
         }
         const unsigned long long wb = wave_min_u64(lbest);
-        if (wb != L2O_SENTINEL) thr = (float)(key_gain(wb) - 1) + slack;
+        if (wb != L2O_SENTINEL) {
+            const float nthr = (float)(key_gain(wb) - 1) + slack;
+            if (nthr != thr) {
+                thr = nthr;
+                __builtin_amdgcn_wave_barrier();
+                for (int t = lane; t < T; t += 64) su[t] = thr + sm.rd[t + 1];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
     };
 
     int t = 0;
-    for (; t < t_full; ++t) {
-        const float4 R = s_rows[t + 1];
-        const float U = thr + R.z;
-        bool hit[K];
+    const int t_main = t_full - t_full % UF;
+    for (; t < t_main; t += UF) {
+        float x[UF], y[UF], U[UF];
+#pragma unroll
+        for (int s = 0; s < UF; ++s) {
+            x[s] = sm.rx[t + 1 + s];
+            y[s] = sm.ry[t + 1 + s];
+            U[s] = su[t + s];
+        }
+        float4 C[UF + K - 1];
+#pragma unroll
+        for (int j = 0; j < UF + K - 1; ++j) C[j] = sm.col[t + 1 + cb + j];
+        bool hit[UF * K];
         bool any = false;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float4 C = s_cols[t + 1 + cb + k];
-            const float dx = R.x - C.x, dy = R.y - C.y;
-            const float q = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
-            const float w2 = (qprev[k] + q) - C.z;
-            qprev[k] = q;
-            hit[k] = w2 <= U;
-            any |= hit[k];
+        for (int s = 0; s < UF; ++s) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const float dx = x[s] - C[s + k].x, dy = y[s] - C[s + k].y;
+                const float q = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+                const float w2 = (qp[k] + q) - C[s + k].z;
+                qp[k] = q;
+                hit[s * K + k] = w2 <= U[s];
+                any |= hit[s * K + k];
+            }
         }
-        if (__ballot(any)) slow(t, hit);
+        if (__ballot(any)) slow(t, UF, hit);
     }
     int lim[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) lim[k] = n - 2 - a0 - (g0 + cb + k);
     for (; t < t_end; ++t) {
-        const float4 R = s_rows[t + 1];
-        const float U = thr + R.z;
+        const float x = sm.rx[t + 1], y = sm.ry[t + 1], U = su[t];
         bool hit[K];
         bool any = false;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float4 C = s_cols[t + 1 + cb + k];
-            const float dx = R.x - C.x, dy = R.y - C.y;
+            const float4 C = sm.col[t + 1 + cb + k];
+            const float dx = x - C.x, dy = y - C.y;
             const float q = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
-            const float w2 = (qprev[k] + q) - C.z;
-            qprev[k] = q;
+            const float w2 = (qp[k] + q) - C.z;
+            qp[k] = q;
             hit[k] = (w2 <= U) && (t <= lim[k]);
             any |= hit[k];
         }
-        if (__ballot(any)) slow(t, hit);
+        if (__ballot(any)) slow(t, 1, hit);
     }
 
     const unsigned long long wb = wave_min_u64(lbest);
     const unsigned wfb = wave_sum_u32(fb);
     if (lane == 0) {
-        s_key[w] = wb;
-        s_fb[w] = wfb;
+        sm.key[w] = wb;
+        sm.fb[w] = wfb;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long key = s_key[0];
-        unsigned f = s_fb[0];
+        unsigned long long key = sm.key[0];
+        unsigned f = sm.fb[0];
 #pragma unroll
         for (int k = 1; k < WAVES; ++k) {
-            key = s_key[k] < key ? s_key[k] : key;
-            f += s_fb[k];
+            key = sm.key[k] < key ? sm.key[k] : key;
+            f += sm.fb[k];
         }
         if (key != L2O_SENTINEL) {
             const unsigned long long cur = __hip_atomic_load(&in->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -491,6 +543,13 @@ __global__ __launch_bounds__(256) void k_sqrt_ulp(uint32_t lo, uint32_t hi, unsi
 
 namespace l2o {
 
+// tile shapes compiled into the library (k diagonals per lane, t steps,
+// waves per workgroup); the autotuner chooses among these per n bucket
+#define L2O_SWEEP_SHAPES(X)                                                                          \
+    X(1, 64, 1) X(1, 128, 1) X(2, 64, 1) X(2, 128, 1) X(4, 64, 1) X(4, 128, 1) X(1, 128, 2)          \
+    X(2, 128, 2) X(4, 128, 2) X(1, 128, 4) X(1, 256, 4) X(2, 128, 4) X(2, 256, 4) X(4, 128, 4)         \
+    X(4, 256, 4)
+
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \
         hipLaunchKernelGGL((k_sweep<K_, T_, W_>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
@@ -501,25 +560,29 @@ hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint3
                         l2o_inst *insts, hipStream_t stream)
 {
     if (nitems == 0) return hipSuccess;
-    L2O_SWEEP_CASE(1, 64, 1)
-    L2O_SWEEP_CASE(1, 128, 1)
-    L2O_SWEEP_CASE(2, 64, 1)
-    L2O_SWEEP_CASE(2, 128, 1)
-    L2O_SWEEP_CASE(1, 128, 4)
-    L2O_SWEEP_CASE(1, 256, 4)
-    L2O_SWEEP_CASE(2, 128, 4)
-    L2O_SWEEP_CASE(2, 256, 4)
-    L2O_SWEEP_CASE(4, 256, 4)
+    L2O_SWEEP_SHAPES(L2O_SWEEP_CASE)
     return hipErrorInvalidValue;
 }
 
+#define L2O_SWEEP_ROW(K_, T_, W_) {K_, T_, W_},
+static const int g_shapes[][3] = {L2O_SWEEP_SHAPES(L2O_SWEEP_ROW)};
+
 bool sweep_cfg_supported(const sweep_cfg &c)
 {
-    static const int table[][3] = {{1, 64, 1}, {1, 128, 1}, {2, 64, 1}, {2, 128, 1}, {1, 128, 4},
-                                   {1, 256, 4}, {2, 128, 4}, {2, 256, 4}, {4, 256, 4}};
-    for (const auto &r : table)
+    for (const auto &r : g_shapes)
         if (r[0] == c.k && r[1] == c.t && r[2] == c.waves) return true;
     return false;
+}
+
+int sweep_shapes(int (*out)[3], int cap)
+{
+    const int m = (int)(sizeof(g_shapes) / sizeof(g_shapes[0]));
+    for (int q = 0; q < m && q < cap; ++q) {
+        out[q][0] = g_shapes[q][0];
+        out[q][1] = g_shapes[q][1];
+        out[q][2] = g_shapes[q][2];
+    }
+    return m;
 }
 
 hipError_t launch_apply(const l2o_apply_item *items, uint32_t nitems, l2o_inst *insts, unsigned *live,

@@ -1,0 +1,284 @@
+// l2o_server.cpp -- continuous-batching request loop over the engine.
+//
+// Requests (a local search or an ILS solve) wait in a FIFO queue on the host.
+// Each pump() admits queued requests into free engine slots while the HBM
+// budget and the slot count allow (engine_add copies coordinates and the start
+// tour to HBM once; they stay resident until the result is harvested), runs a
+// bounded number of batched steps in which every admitted request advances
+// together, and harvests the requests that finished.  A request that arrives
+// while others are on the GPU therefore joins the very next batch of steps.
+// Admission is FIFO: when the head of the queue does not fit, later requests
+// wait too (no overtaking), so latency order follows arrival order.
+//
+// The reference has no serving layer; its closest relative is the multi-start
+// ILS driver with one solver per pthread (src/solvers/ILSGlobalSolverMT.cpp:86-195).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/logo2opt.h"
+
+namespace {
+
+double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+enum req_state { Q_QUEUED = 0, Q_RUNNING = 1, Q_DONE = 2, Q_REJECTED = 3 };
+
+struct request {
+    int64_t id = 0;
+    uint32_t n = 0;
+    std::vector<logo2opt_city_coords> coords;
+    std::vector<uint32_t> route;
+    logo2opt_job job{};
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int32_t slot = -1;
+    int state = Q_QUEUED;
+    logo2opt_result res{};
+};
+
+}  // namespace
+
+struct logo2opt_server {
+    logo2opt_engine *e = nullptr;
+    uint32_t queue_cap = 0;
+    int64_t next_id = 1;
+    std::deque<int64_t> queue;
+    std::map<int64_t, request> reqs;
+    std::map<int32_t, int64_t> slot_req;
+    std::vector<int64_t> completed;  // not yet polled
+    uint64_t n_submitted = 0, n_rejected = 0, n_admitted = 0, n_done = 0;
+};
+
+extern "C" {
+
+int logo2opt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots, uint32_t queue_cap,
+                           logo2opt_server **out)
+{
+    if (!out || max_slots == 0) return LOGO2OPT_EINVAL;
+    logo2opt_server *s = new (std::nothrow) logo2opt_server();
+    if (!s) return LOGO2OPT_ENOMEM;
+    int rc = logo2opt_engine_create(device, hbm_budget_bytes, max_slots, &s->e);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    s->queue_cap = queue_cap ? queue_cap : 1u << 20;
+    *out = s;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_destroy(logo2opt_server *s)
+{
+    if (!s) return LOGO2OPT_EINVAL;
+    if (s->e) logo2opt_engine_destroy(s->e);
+    delete s;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
+                           const logo2opt_job *job, int64_t *id)
+{
+    if (!s || !coords || !route || !id || n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
+    s->n_submitted++;
+    if (s->queue.size() >= s->queue_cap) {
+        s->n_rejected++;
+        *id = -1;
+        return LOGO2OPT_EAGAIN;
+    }
+    request r;
+    r.id = s->next_id++;
+    r.n = n;
+    r.coords.assign(coords, coords + n);
+    r.route.assign(route, route + n);
+    if (job) r.job = *job;
+    r.t_submit = now_s();
+    *id = r.id;
+    s->queue.push_back(r.id);
+    s->reqs.emplace(r.id, std::move(r));
+    return LOGO2OPT_OK;
+}
+
+// admit -> run up to `steps` batched steps -> harvest
+int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every, uint32_t *running,
+                         uint32_t *queued)
+{
+    if (!s) return LOGO2OPT_EINVAL;
+    while (!s->queue.empty()) {
+        request &r = s->reqs[s->queue.front()];
+        int32_t slot = -1;
+        const int rc = logo2opt_engine_add(s->e, r.n, r.coords.data(), r.route.data(), &r.job, &slot);
+        if (rc == LOGO2OPT_EAGAIN) break;  // head of line does not fit yet
+        s->queue.pop_front();
+        if (rc) {  // invalid request: finished with an error
+            r.state = Q_REJECTED;
+            r.t_done = now_s();
+            s->n_rejected++;
+            continue;
+        }
+        r.slot = slot;
+        r.state = Q_RUNNING;
+        r.t_admit = now_s();
+        s->slot_req[slot] = r.id;
+        s->n_admitted++;
+        // host copies are no longer needed: the instance is resident in HBM
+        std::vector<logo2opt_city_coords>().swap(r.coords);
+    }
+    uint32_t live = 0;
+    if (!s->slot_req.empty()) {
+        const int rc = logo2opt_engine_run(s->e, steps, sync_every, &live);
+        if (rc) return rc;
+    }
+    int32_t done[256];
+    for (;;) {
+        uint32_t cnt = 0;
+        int rc = logo2opt_engine_poll_done(s->e, done, 256, &cnt);
+        if (rc) return rc;
+        if (cnt == 0) break;
+        const double t = now_s();
+        for (uint32_t q = 0; q < cnt; ++q) {
+            auto it = s->slot_req.find(done[q]);
+            if (it == s->slot_req.end()) continue;
+            request &r = s->reqs[it->second];
+            rc = logo2opt_engine_result(s->e, done[q], r.route.data(), &r.res, nullptr);
+            if (rc) return rc;
+            rc = logo2opt_engine_remove(s->e, done[q]);
+            if (rc) return rc;
+            r.state = Q_DONE;
+            r.t_done = t;
+            r.slot = -1;
+            s->slot_req.erase(it);
+            s->completed.push_back(r.id);
+            s->n_done++;
+        }
+    }
+    if (running) *running = (uint32_t)s->slot_req.size();
+    if (queued) *queued = (uint32_t)s->queue.size();
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_poll(logo2opt_server *s, int64_t *ids, uint32_t cap, uint32_t *count)
+{
+    if (!s || !count) return LOGO2OPT_EINVAL;
+    const uint32_t c = (uint32_t)std::min<size_t>(cap, s->completed.size());
+    for (uint32_t q = 0; q < c; ++q) ids[q] = s->completed[q];
+    s->completed.erase(s->completed.begin(), s->completed.begin() + c);
+    *count = c;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_result(logo2opt_server *s, int64_t id, uint32_t *route_out, logo2opt_result *res,
+                           double *latency_s, double *queue_s)
+{
+    if (!s) return LOGO2OPT_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return LOGO2OPT_ENOTFOUND;
+    request &r = it->second;
+    if (r.state != Q_DONE) return LOGO2OPT_ESTATE;
+    if (route_out) std::memcpy(route_out, r.route.data(), 4ull * r.n);
+    if (res) *res = r.res;
+    if (latency_s) *latency_s = r.t_done - r.t_submit;
+    if (queue_s) *queue_s = r.t_admit - r.t_submit;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_release(logo2opt_server *s, int64_t id)
+{
+    if (!s) return LOGO2OPT_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return LOGO2OPT_ENOTFOUND;
+    if (it->second.state == Q_RUNNING) {
+        const int rc = logo2opt_engine_remove(s->e, it->second.slot);
+        if (rc) return rc;
+        s->slot_req.erase(it->second.slot);
+    } else if (it->second.state == Q_QUEUED) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+    }
+    s->reqs.erase(it);
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_counts(logo2opt_server *s, uint64_t *counts)
+{
+    if (!s || !counts) return LOGO2OPT_EINVAL;
+    counts[0] = s->n_submitted;
+    counts[1] = s->n_rejected;
+    counts[2] = s->n_admitted;
+    counts[3] = s->n_done;
+    counts[4] = s->queue.size();
+    counts[5] = s->slot_req.size();
+    return LOGO2OPT_OK;
+}
+
+// Open-loop replay: request q arrives arrival_s[q] seconds after the start
+// (ascending) and is submitted then; the loop pumps `steps` steps at a time
+// until every request finished or `horizon_s` elapsed.  latency_out[q] is the
+// time from the scheduled arrival to completion (-1: rejected or unfinished),
+// queue_out[q] the time from arrival to admission.
+int logo2opt_server_replay(logo2opt_server *s, uint32_t count, const double *arrival_s, const uint32_t *n,
+                           const logo2opt_city_coords *const *coords, const uint32_t *const *routes,
+                           const logo2opt_job *jobs, double horizon_s, uint32_t steps, uint32_t sync_every,
+                           double *latency_out, double *queue_out, double *elapsed_s)
+{
+    if (!s || !arrival_s || !n || !coords || !routes || !jobs || !latency_out) return LOGO2OPT_EINVAL;
+    std::vector<int64_t> ids(count, -1);
+    std::map<int64_t, uint32_t> idx;
+    for (uint32_t q = 0; q < count; ++q) {
+        latency_out[q] = -1.0;
+        if (queue_out) queue_out[q] = -1.0;
+    }
+    const double t0 = now_s();
+    uint32_t next = 0, finished = 0;
+    while (true) {
+        const double t = now_s() - t0;
+        if (t > horizon_s) break;
+        while (next < count && arrival_s[next] <= t) {
+            int64_t id = -1;
+            const int rc = logo2opt_server_submit(s, n[next], coords[next], routes[next], &jobs[next], &id);
+            if (rc == LOGO2OPT_OK) {
+                ids[next] = id;
+                idx[id] = next;
+            } else if (rc != LOGO2OPT_EAGAIN) {
+                return rc;
+            } else {
+                ++finished;  // rejected at the door (queue full)
+            }
+            ++next;
+        }
+        uint32_t running = 0, queued = 0;
+        int rc = logo2opt_server_pump(s, steps, sync_every, &running, &queued);
+        if (rc) return rc;
+        int64_t done[256];
+        for (;;) {
+            uint32_t c = 0;
+            rc = logo2opt_server_poll(s, done, 256, &c);
+            if (rc) return rc;
+            if (!c) break;
+            for (uint32_t k = 0; k < c; ++k) {
+                auto it = idx.find(done[k]);
+                if (it == idx.end()) continue;
+                const request &r = s->reqs[done[k]];
+                latency_out[it->second] = (r.t_done - t0) - arrival_s[it->second];
+                if (queue_out) queue_out[it->second] = (r.t_admit - t0) - arrival_s[it->second];
+                ++finished;
+            }
+        }
+        if (finished >= count && next >= count) break;
+        if (running == 0 && queued == 0 && next < count) {
+            const double wait = arrival_s[next] - (now_s() - t0);
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+        }
+    }
+    if (elapsed_s) *elapsed_s = now_s() - t0;
+    return LOGO2OPT_OK;
+}
+
+}  // extern "C"

@@ -22,8 +22,9 @@ EXPORTS = [
     "logo2opt_ils", "logo2opt_route_length", "logo2opt_get_stats", "logo2opt_set_timing", "logo2opt_time_sweep",
     "logo2opt_engine_create", "logo2opt_engine_destroy", "logo2opt_engine_bytes_for", "logo2opt_engine_add",
     "logo2opt_engine_run", "logo2opt_engine_result", "logo2opt_engine_round_lengths", "logo2opt_engine_remove",
-    "logo2opt_engine_poll_done", "logo2opt_engine_stats", "logo2opt_engine_set_timing", "logo2opt_gen_uniform", "logo2opt_gen_clustered",
+    "logo2opt_engine_poll_done", "logo2opt_engine_stats", "logo2opt_engine_set_timing", "logo2opt_engine_set_graphs", "logo2opt_gen_uniform", "logo2opt_gen_clustered",
     "logo2opt_random_perm", "logo2opt_nn_tour", "logo2opt_xy_to_coords", "logo2opt_sqrt_ulp_check",
+    "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
 ]
 
 
@@ -89,6 +90,7 @@ def load():
     L.logo2opt_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.logo2opt_set_timing.argtypes = [vp, C.c_int]
     L.logo2opt_engine_set_timing.argtypes = [vp, C.c_int]
+    L.logo2opt_engine_set_graphs.argtypes = [vp, C.c_int]
     L.logo2opt_time_sweep.argtypes = [vp, u32p, C.c_uint32, C.POINTER(C.c_double)]
     L.logo2opt_engine_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(vp)]
     L.logo2opt_engine_destroy.argtypes = [vp]
@@ -112,6 +114,10 @@ def load():
     L.logo2opt_xy_to_coords.argtypes = [C.c_uint32, i32p, vp]
     L.logo2opt_xy_to_coords.restype = None
     L.logo2opt_sqrt_ulp_check.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_tuning_shapes.argtypes = [i32p, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_tuning_set.argtypes = [u32p, i32p, C.c_uint32]
+    L.logo2opt_tuning_get.argtypes = [u32p, i32p, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_tuning_load.argtypes = [C.c_char_p]
     _lib = L
     return L
 
@@ -175,6 +181,33 @@ def sqrt_ulp_check(lo_bits, hi_bits, device=0):
     m = C.c_uint32()
     _check(load().logo2opt_sqrt_ulp_check(device, lo_bits, hi_bits, C.byref(m)), "sqrt_ulp_check")
     return m.value
+
+
+# ---- sweep tile shapes -------------------------------------------------------
+def tuning_shapes():
+    out = np.zeros(3 * 64, np.int32)
+    cnt = C.c_uint32()
+    _check(load().logo2opt_tuning_shapes(out, 64, C.byref(cnt)), "tuning_shapes")
+    return [tuple(int(v) for v in out[3 * q:3 * q + 3]) for q in range(cnt.value)]
+
+
+def tuning_set(rows):
+    """rows: [(max_n, (k, t, waves)), ...]"""
+    mx = np.array([r[0] for r in rows], np.uint32)
+    sh = np.array([v for r in rows for v in r[1]], np.int32)
+    _check(load().logo2opt_tuning_set(mx, sh, len(rows)), "tuning_set")
+
+
+def tuning_get():
+    mx = np.zeros(64, np.uint32)
+    sh = np.zeros(3 * 64, np.int32)
+    cnt = C.c_uint32()
+    _check(load().logo2opt_tuning_get(mx, sh, 64, C.byref(cnt)), "tuning_get")
+    return [(int(mx[q]), tuple(int(v) for v in sh[3 * q:3 * q + 3])) for q in range(cnt.value)]
+
+
+def tuning_load(path):
+    _check(load().logo2opt_tuning_load(path.encode()), "tuning_load")
 
 
 # ---- single-instance context (the CUDASolver replacement) -----------------
@@ -334,7 +367,110 @@ class Engine:
     def set_timing(self, on):
         _check(load().logo2opt_engine_set_timing(self.h, 1 if on else 0), "logo2opt_engine_set_timing")
 
+    def set_graphs(self, on):
+        _check(load().logo2opt_engine_set_graphs(self.h, 1 if on else 0), "logo2opt_engine_set_graphs")
+
     def stats(self):
         s = Stats()
         _check(load().logo2opt_engine_stats(self.h, C.byref(s)), "logo2opt_engine_stats")
         return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+
+# ---- server (continuous batching of requests) -------------------------------
+SERVER_EXPORTS = ["logo2opt_server_create", "logo2opt_server_destroy", "logo2opt_server_submit",
+                  "logo2opt_server_pump", "logo2opt_server_poll", "logo2opt_server_result",
+                  "logo2opt_server_release", "logo2opt_server_counts", "logo2opt_server_replay"]
+EXPORTS += SERVER_EXPORTS
+
+
+def _bind_server(L):
+    if getattr(L, "_server_bound", False):
+        return L
+    L.logo2opt_server_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(vp)]
+    L.logo2opt_server_destroy.argtypes = [vp]
+    L.logo2opt_server_submit.argtypes = [vp, C.c_uint32, vp, u32p, C.POINTER(Job), C.POINTER(C.c_int64)]
+    L.logo2opt_server_pump.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.logo2opt_server_poll.argtypes = [vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_server_result.argtypes = [vp, C.c_int64, vp, C.POINTER(Result), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
+    L.logo2opt_server_release.argtypes = [vp, C.c_int64]
+    L.logo2opt_server_counts.argtypes = [vp, u64p]
+    L.logo2opt_server_replay.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_double, C.c_uint32, C.c_uint32,
+                                         vp, vp, C.POINTER(C.c_double)]
+    L._server_bound = True
+    return L
+
+
+class Server:
+    """Continuous-batching request loop (logo2opt_server_*)."""
+
+    def __init__(self, device=0, hbm_budget=4 << 30, max_slots=1024, queue_cap=0):
+        L = _bind_server(load())
+        h = vp()
+        _check(L.logo2opt_server_create(device, hbm_budget, max_slots, queue_cap, C.byref(h)),
+               "logo2opt_server_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().logo2opt_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0):
+        route = np.ascontiguousarray(route, np.uint32)
+        j = Job(ils_rounds, kick, seed, 0, 0)
+        rid = C.c_int64()
+        rc = load().logo2opt_server_submit(self.h, len(route), C.cast(coords, vp), route, C.byref(j), C.byref(rid))
+        if rc == -6:
+            return None
+        _check(rc, "logo2opt_server_submit")
+        return rid.value
+
+    def pump(self, steps=64, sync_every=16):
+        run, q = C.c_uint32(), C.c_uint32()
+        _check(load().logo2opt_server_pump(self.h, steps, sync_every, C.byref(run), C.byref(q)), "server_pump")
+        return run.value, q.value
+
+    def poll(self, cap=1024):
+        buf = (C.c_int64 * cap)()
+        cnt = C.c_uint32()
+        _check(load().logo2opt_server_poll(self.h, C.cast(buf, vp), cap, C.byref(cnt)), "server_poll")
+        return [buf[k] for k in range(cnt.value)]
+
+    def result(self, rid, n):
+        r = np.zeros(n, np.uint32)
+        res = Result()
+        lat, qs = C.c_double(), C.c_double()
+        _check(load().logo2opt_server_result(self.h, rid, r.ctypes.data, C.byref(res), C.byref(lat), C.byref(qs)),
+               "server_result")
+        out = {f: getattr(res, f) for f, _ in Result._fields_}
+        out.update(route=r, latency_s=lat.value, queue_s=qs.value)
+        return out
+
+    def counts(self):
+        c = np.zeros(6, np.uint64)
+        _check(load().logo2opt_server_counts(self.h, c), "server_counts")
+        return dict(zip(["submitted", "rejected", "admitted", "done", "queued", "running"], c.tolist()))
+
+    def replay(self, arrivals, requests, horizon_s, steps=32, sync_every=16):
+        """requests: list of (coords, route, Job).  Returns (latency[], queue[], elapsed)."""
+        m = len(requests)
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        ns = np.array([len(r[1]) for r in requests], np.uint32)
+        cptr = (vp * m)(*[C.cast(r[0], vp) for r in requests])
+        routes = [np.ascontiguousarray(r[1], np.uint32) for r in requests]
+        rptr = (vp * m)(*[r.ctypes.data for r in routes])
+        jobs = (Job * m)(*[r[2] for r in requests])
+        lat = np.zeros(m, np.float64)
+        qs = np.zeros(m, np.float64)
+        el = C.c_double()
+        _check(load().logo2opt_server_replay(self.h, m, arr.ctypes.data, ns.ctypes.data, C.cast(cptr, vp),
+                                             C.cast(rptr, vp), C.cast(jobs, vp), horizon_s, steps, sync_every,
+                                             lat.ctypes.data, qs.ctypes.data, C.byref(el)), "server_replay")
+        return lat, qs, el.value
