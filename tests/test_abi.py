@@ -91,3 +91,27 @@ def test_invalid_inputs_rejected_before_device_work():
 def test_engine_bytes_for():
     assert L.Engine.bytes_for(1000) == 2 * 16128 + 2 * 4096
     assert L.Engine.bytes_for(1000, ils_rounds=10) > L.Engine.bytes_for(1000)
+
+
+def test_header_is_c_and_cpp_adapter_compiles(tmp_path):
+    pkg = os.path.dirname(L.LIB_PATH)
+    inc = os.path.join(ROOT, "include")
+    c_src = tmp_path / "t.c"
+    c_src.write_text('#include "logo2opt.h"\n#include <string.h>\n'
+                     'int main(void) { return strcmp(logo2opt_strerror(LOGO2OPT_ETOUR),\n'
+                     '                               "route is not a permutation of 0..n-1") != 0; }\n')
+    exe = tmp_path / "tc"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, str(c_src), "-o", str(exe), "-L", pkg,
+                    "-llogo2opt", f"-Wl,-rpath,{pkg}"], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
+    cpp_src = tmp_path / "t.cpp"
+    cpp_src.write_text('#include "hip_local_solver.h"\n#include <cstdio>\n'
+                       'int main() { try { logo2opt_city_coords c[2] = {{0, 0}, {1, 0}};\n'
+                       '  logo2opt::HIPSolver s(c, 2, 0); } catch (const std::runtime_error &e) {\n'
+                       '  std::puts(e.what()); return 0; } return 1; }\n')
+    exe2 = tmp_path / "tcpp"
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-I", inc, "-I", os.path.join(pkg, "csrc"),
+                    str(cpp_src), "-o", str(exe2), "-L", pkg, "-llogo2opt", f"-Wl,-rpath,{pkg}"], check=True)
+    r = subprocess.run([str(exe2)], capture_output=True, text=True)
+    # n = 2 is rejected before any device work: the adapter throws, never exits
+    assert r.returncode == 0 and "invalid argument" in r.stdout
