@@ -186,6 +186,22 @@ def config3(rounds, start, seed=3):
             "evals_per_s": st["evals"] / dt, "fallbacks": st["fallbacks"]}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device hipMemcpy of nbytes (a plain copy kernel reading
+    and writing nbytes each): the yardstick for FETCH_SIZE / WRITE_SIZE."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +220,12 @@ def main():
     ap.add_argument("--config3-rounds", type=int, default=500)
     ap.add_argument("--config3-start", default="nn", choices=["nn", "random"])
     ap.add_argument("--no-graph", action="store_true", help="direct launches instead of HIP graph replay")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration << 20)
+        return
     if args.no_graph:
         os.environ["LOGO2OPT_GRAPHS"] = "0"
     if args.gpus != 1:

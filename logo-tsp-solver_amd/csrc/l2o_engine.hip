@@ -222,7 +222,7 @@ struct logo2opt_engine {
     // work lists
     struct group {
         sweep_cfg cfg;
-        uint32_t off, count;
+        uint32_t off, count, id;
     };
     std::vector<group> groups;
     l2o_sweep_item *d_sitems = nullptr;
@@ -386,9 +386,15 @@ int rebuild_lists(logo2opt_engine *e)
     std::vector<l2o_apply_item> ai;
     e->ils_grid = 0;
     e->any_ils = false;
-    for (uint32_t id = 0; id < e->max_slots; ++id) {
+    // largest instances first: their long sweeps start early and the small
+    // ones fill the tail (longest-processing-time order)
+    std::vector<uint32_t> order;
+    for (uint32_t id = 0; id < e->max_slots; ++id)
+        if (e->slots[id].used && !e->slots[id].done) order.push_back(id);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return e->slots[a].n > e->slots[b].n; });
+    for (uint32_t id : order) {
         const slot_t &s = e->slots[id];
-        if (!s.used || s.done) continue;
         const uint32_t n = s.n;
         const sweep_cfg c = cfg_for(n);
         auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves)];
@@ -407,22 +413,15 @@ int rebuild_lists(logo2opt_engine *e)
     e->groups.clear();
     for (auto &kv : by_cfg) {
         const sweep_cfg c{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
-        // XCD-aware order: workgroup b is dispatched to XCD b % 8, so give
-        // XCD x a contiguous share of the (instance-ordered) item list; its L2
-        // then holds only the points of its own instances and tiles.  Shares
-        // differ by at most one item, which keeps b % 8 aligned to the share.
+        // Persistent workgroups take items in list order from a ticket
+        // counter, so at any moment the whole device works on a window of
+        // consecutive items of one or a few instances (largest first), whose
+        // points stay hot in every XCD's L2.
         const auto &v = kv.second;
         const uint32_t cnt = (uint32_t)v.size();
-        const uint32_t X = 8, base = cnt / X, rem = cnt % X;
-        std::vector<l2o_sweep_item> perm;
-        perm.reserve(cnt);
-        for (uint32_t r = 0; r <= base; ++r)
-            for (uint32_t x = 0; x < X; ++x) {
-                if (r == base && x >= rem) break;
-                perm.push_back(v[x * base + std::min(x, rem) + r]);
-            }
-        e->groups.push_back({c, (uint32_t)all.size(), cnt});
-        all.insert(all.end(), perm.begin(), perm.end());
+        if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;
+        e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size()});
+        all.insert(all.end(), v.begin(), v.end());
     }
     if (all.size() > e->sitems_cap) {
         if (e->d_sitems) (void)hipFree(e->d_sitems);

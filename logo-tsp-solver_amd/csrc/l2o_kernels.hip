@@ -30,6 +30,10 @@
 // (the reference's grid tie-break is a racy lock, opt_kernel.cu:239-261).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "l2o_state.h"
 
 namespace l2o {
@@ -98,18 +102,17 @@ struct sweep_smem {
     unsigned fb[WAVES];
 };
 
+// one work item: all candidates of diagonals [g0, g0+DIAG) x steps [a0, a0+T)
 template <int K, int T, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__restrict__ items,
-                                                      l2o_inst *__restrict__ insts)
+__device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__restrict__ insts,
+                                           sweep_smem<K, T, WAVES> &sm)
 {
     constexpr int NT = 64 * WAVES;
     constexpr int DIAG = NT * K;
     constexpr int NCOL = T + DIAG;
     constexpr int UF = 4;  // steps per unrolled iteration
     static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
-    __shared__ sweep_smem<K, T, WAVES> sm;
 
-    const l2o_sweep_item it = items[blockIdx.x];
     l2o_inst *in = insts + it.inst;
     if (in->phase != L2O_LO) return;
     const int n = (int)in->n;
@@ -260,6 +263,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
         }
         if (f) atomicAdd(&in->fallbacks, (unsigned long long)f);
     }
+}
+
+// One workgroup per work item.  (A persistent variant that loops over items
+// fetched from an atomic ticket is NOT used: with a lane-0 ticket fetch and
+// lane-0 key atomics inside the item loop, the AMDGPU control-flow
+// structurizer emits an exec-masked loop nest in which lanes 1..63 re-enter
+// the item body on a stale ticket while lane 0 waits outside -- a livelock.
+// Dispatch-round quantization is handled instead by short items, t = 64.)
+template <int K, int T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__restrict__ items,
+                                                      l2o_inst *__restrict__ insts)
+{
+    __shared__ sweep_smem<K, T, WAVES> sm;
+    sweep_item<K, T, WAVES>(items[blockIdx.x], insts, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,8 +564,8 @@ namespace l2o {
 // waves per workgroup); the autotuner chooses among these per n bucket
 #define L2O_SWEEP_SHAPES(X)                                                                          \
     X(1, 64, 1) X(1, 128, 1) X(2, 64, 1) X(2, 128, 1) X(4, 64, 1) X(4, 128, 1) X(1, 128, 2)          \
-    X(2, 128, 2) X(4, 128, 2) X(1, 128, 4) X(1, 256, 4) X(2, 128, 4) X(2, 256, 4) X(4, 128, 4)         \
-    X(4, 256, 4)
+    X(2, 64, 2) X(2, 128, 2) X(4, 64, 2) X(4, 128, 2) X(1, 64, 4) X(1, 128, 4) X(1, 256, 4)            \
+    X(2, 64, 4) X(2, 128, 4) X(2, 256, 4) X(4, 64, 4) X(4, 128, 4) X(4, 256, 4)
 
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \

@@ -18,6 +18,7 @@ struct sweep_cfg {
 bool sweep_cfg_supported(const sweep_cfg &c);
 // compiled shapes: fills up to cap rows {k, t, waves}, returns the total count
 int sweep_shapes(int (*out)[3], int cap);
+// one workgroup per item
 hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint32_t nitems,
                         l2o_inst *insts, hipStream_t stream);
 hipError_t launch_apply(const l2o_apply_item *items, uint32_t nitems, l2o_inst *insts, unsigned *live,

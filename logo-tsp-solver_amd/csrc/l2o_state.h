@@ -78,6 +78,7 @@ struct l2o_apply_item {
     uint32_t chunk;
 };
 
-#define L2O_APPLY_CHUNK 4096
+#define L2O_MAX_GROUPS 32  // distinct sweep tile shapes in one engine step
+#define L2O_APPLY_CHUNK 1024
 #define L2O_APPLY_THREADS 256
 #define L2O_ILS_THREADS 1024
