@@ -133,7 +133,8 @@ def batched_vs_plain(count, plain_every, seed=4):
     }
 
 
-def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=512):
+def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=512, max_running=0,
+            max_inflight_pairs=0):
     """Open-loop Poisson replay through the continuous-batching server."""
     rs = np.random.default_rng(seed)
     insts = []
@@ -149,6 +150,7 @@ def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=51
         pick = rs.integers(0, pool, len(arrivals))
         reqs = [(insts[p][1], insts[p][2], L.Job(rounds, 0, 7000 + q, 0, 0)) for q, p in enumerate(pick)]
         with L.Server(hbm_budget=8 << 30, max_slots=max_slots, queue_cap=1 << 20) as srv:
+            srv.set_limits(max_running, max_inflight_pairs)
             lat, qs, el = srv.replay(arrivals, reqs, horizon_s, steps=32, sync_every=16)
             cnt = srv.counts()
         ok = lat >= 0
@@ -167,7 +169,8 @@ def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=51
     return {"workload": f"ILS requests ({rounds} double-bridge rounds, greedy NN start), n log-uniform in "
                         f"[200,50000], uniform or clustered coords in [0,1e6)^2 (pool of {pool} instances, "
                         f"distinct kick seeds), Poisson arrivals for {window_s}s, horizon {horizon_s}s",
-            "max_slots": max_slots, "rates": out}
+            "max_slots": max_slots, "max_running": max_running, "max_inflight_pairs": max_inflight_pairs,
+            "rates": out}
 
 
 def config3(rounds, start, seed=3):
@@ -216,6 +219,8 @@ def main():
     ap.add_argument("--serve-rates", default="50")
     ap.add_argument("--serve-window", type=float, default=4.0)
     ap.add_argument("--serve-horizon", type=float, default=12.0)
+    ap.add_argument("--serve-max-running", type=int, default=0)
+    ap.add_argument("--serve-max-pairs", type=float, default=0, help="cap on in-flight candidates per sweep")
     ap.add_argument("--config3", action="store_true", help="also run the 100k-city ILS (long)")
     ap.add_argument("--config3-rounds", type=int, default=500)
     ap.add_argument("--config3-start", default="nn", choices=["nn", "random"])
@@ -291,7 +296,8 @@ def main():
     if not args.skip_serve:
         log("serving")
         rates = [float(v) for v in args.serve_rates.split(",")]
-        res["serving"] = serving(rates, args.serve_window, args.serve_horizon)
+        res["serving"] = serving(rates, args.serve_window, args.serve_horizon, max_running=args.serve_max_running,
+                                 max_inflight_pairs=int(args.serve_max_pairs))
     if args.config3:
         log("config3 ILS 100k")
         res["ils_100k"] = config3(args.config3_rounds, args.config3_start)

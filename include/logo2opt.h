@@ -216,6 +216,11 @@ int logo2opt_server_result(logo2opt_server *s, int64_t id, uint32_t *route_out, 
                            double *latency_s, double *queue_s);
 /* Forget a request (cancels it if queued or running). */
 int logo2opt_server_release(logo2opt_server *s, int64_t id);
+/* Admission limits besides HBM and slots (0 = none): at most max_running
+ * requests at once, and at most max_inflight_pairs candidate moves per sweep
+ * summed over running requests (the head of the queue is always admitted
+ * when nothing runs). */
+int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_t max_inflight_pairs);
 /* counts[6] = {submitted, rejected, admitted, done, queued now, running now} */
 int logo2opt_server_counts(logo2opt_server *s, uint64_t *counts);
 /* Open-loop replay of `count` requests arriving at arrival_s[q] seconds
@@ -237,13 +242,19 @@ int logo2opt_server_replay(logo2opt_server *s, uint32_t count, const double *arr
  * n bucket from a table written by the autotuner (tune.py). */
 /* Shapes compiled into the library (3 ints per row); count = total rows. */
 int logo2opt_tuning_shapes(int32_t *out, uint32_t cap, uint32_t *count);
-/* Replace the table: row r applies to n <= max_n[r] (the last row to all
- * larger n).  Affects instances admitted afterwards. */
-int logo2opt_tuning_set(const uint32_t *max_n, const int32_t *shapes, uint32_t rows);
-int logo2opt_tuning_get(uint32_t *max_n, int32_t *shapes, uint32_t cap, uint32_t *rows);
-/* Load a table file ("max_n k t waves" per line).  The file named by the
- * LOGO2OPT_TUNING environment variable is loaded on first use. */
-int logo2opt_tuning_load(const char *path);
+/* Two tables: table 0 is used while an engine holds fewer than 8 live
+ * instances (latency), table 1 for larger batches (throughput).  Replace a
+ * table: row r applies to n <= max_n[r] (the last row to all larger n); work
+ * lists built afterwards use it. */
+int logo2opt_tuning_set(int table, const uint32_t *max_n, const int32_t *shapes, uint32_t rows);
+int logo2opt_tuning_get(int table, uint32_t *max_n, int32_t *shapes, uint32_t cap, uint32_t *rows);
+/* Load a table file ("max_n k t waves" per line).  On first use the library
+ * loads $LOGO2OPT_TUNING / $LOGO2OPT_TUNING_BATCH, else tuning_gfx950.txt /
+ * tuning_batch_gfx950.txt next to itself. */
+int logo2opt_tuning_load(int table, const char *path);
+/* Times `sweeps` launches of the sweep kernels over every live instance of
+ * the engine (no apply), HIP events on the engine stream. */
+int logo2opt_engine_time_sweep(logo2opt_engine *e, uint32_t sweeps, double *avg_ms);
 
 /* ======================================================================== */
 /* Synthetic workloads and construction (host, deterministic)               */

@@ -54,13 +54,19 @@ struct tune_row {
     sweep_cfg cfg;
 };
 
+// Two tables: 0 = one instance in flight (latency: small tiles fill the
+// device), 1 = a batch (throughput: the batch fills the device, larger tiles
+// waste less).  An engine uses table 1 when it holds L2O_BATCH_MIN or more
+// live instances.
+#define L2O_BATCH_MIN 8
 std::mutex g_tune_mu;
-std::vector<tune_row> g_tune = {
-    {640, {1, 64, 1}},
-    {8192, {1, 128, 4}},
-    {LOGO2OPT_MAX_N, {2, 256, 4}},
+std::vector<tune_row> g_tune[2] = {
+    {{640, {1, 64, 1}}, {8192, {1, 128, 4}}, {LOGO2OPT_MAX_N, {2, 256, 4}}},
+    {{640, {1, 64, 1}}, {8192, {1, 128, 4}}, {LOGO2OPT_MAX_N, {2, 256, 4}}},
 };
-bool g_tune_loaded = false;
+bool g_tune_loaded[2] = {false, false};
+const char *g_tune_file[2] = {"tuning_gfx950.txt", "tuning_batch_gfx950.txt"};
+const char *g_tune_env[2] = {"LOGO2OPT_TUNING", "LOGO2OPT_TUNING_BATCH"};
 
 // text table, one "max_n k t waves" row per line; returns rows accepted
 int read_tuning_file(const char *path, std::vector<tune_row> &rows)
@@ -81,43 +87,43 @@ int read_tuning_file(const char *path, std::vector<tune_row> &rows)
     return (int)rows.size();
 }
 
-void install_rows_locked(std::vector<tune_row> rows)
+void install_rows_locked(int table, std::vector<tune_row> rows)
 {
     std::sort(rows.begin(), rows.end(), [](const tune_row &a, const tune_row &b) { return a.max_n < b.max_n; });
     rows.back().max_n = LOGO2OPT_MAX_N;
-    g_tune = rows;
+    g_tune[table] = rows;
 }
 
-// $LOGO2OPT_TUNING, else tuning_gfx950.txt next to this library (written by
-// tune.py), else the built-in defaults above
-void load_tuning_locked()
+// $LOGO2OPT_TUNING[_BATCH], else the table file next to this library
+// (written by tune.py), else the built-in defaults above
+void load_tuning_locked(int table)
 {
-    if (g_tune_loaded) return;
-    g_tune_loaded = true;
+    if (g_tune_loaded[table]) return;
+    g_tune_loaded[table] = true;
     std::string path;
-    const char *p = std::getenv("LOGO2OPT_TUNING");
+    const char *p = std::getenv(g_tune_env[table]);
     if (p && *p) {
         path = p;
     } else {
         Dl_info info;
-        if (dladdr((void *)&load_tuning_locked, &info) && info.dli_fname) {
+        if (dladdr((void *)&read_tuning_file, &info) && info.dli_fname) {
             path = info.dli_fname;
             const size_t slash = path.rfind('/');
-            path = (slash == std::string::npos ? std::string() : path.substr(0, slash + 1)) + "tuning_gfx950.txt";
+            path = (slash == std::string::npos ? std::string() : path.substr(0, slash + 1)) + g_tune_file[table];
         }
     }
     if (path.empty()) return;
     std::vector<tune_row> rows;
-    if (read_tuning_file(path.c_str(), rows) > 0) install_rows_locked(rows);
+    if (read_tuning_file(path.c_str(), rows) > 0) install_rows_locked(table, rows);
 }
 
-sweep_cfg cfg_for(uint32_t n)
+sweep_cfg cfg_for(uint32_t n, int table)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    load_tuning_locked();
-    for (const auto &r : g_tune)
+    load_tuning_locked(table);
+    for (const auto &r : g_tune[table])
         if (n <= r.max_n) return r.cfg;
-    return g_tune.back().cfg;
+    return g_tune[table].back().cfg;
 }
 
 // ---------------------------------------------------------------------------
@@ -393,10 +399,11 @@ int rebuild_lists(logo2opt_engine *e)
         if (e->slots[id].used && !e->slots[id].done) order.push_back(id);
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return e->slots[a].n > e->slots[b].n; });
+    const int table = order.size() >= L2O_BATCH_MIN ? 1 : 0;
     for (uint32_t id : order) {
         const slot_t &s = e->slots[id];
         const uint32_t n = s.n;
-        const sweep_cfg c = cfg_for(n);
+        const sweep_cfg c = cfg_for(n, table);
         auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves)];
         for (uint32_t g0 = 2; g0 + 2 <= n; g0 += c.diag()) {
             const uint32_t len = n - 1 - g0;
@@ -740,9 +747,9 @@ int logo2opt_tuning_shapes(int32_t *out, uint32_t cap, uint32_t *count)
     return LOGO2OPT_OK;
 }
 
-int logo2opt_tuning_set(const uint32_t *max_n, const int32_t *shapes, uint32_t rows)
+int logo2opt_tuning_set(int table, const uint32_t *max_n, const int32_t *shapes, uint32_t rows)
 {
-    if (!max_n || !shapes || rows == 0) return LOGO2OPT_EINVAL;
+    if (table < 0 || table > 1 || !max_n || !shapes || rows == 0) return LOGO2OPT_EINVAL;
     std::vector<tune_row> v;
     for (uint32_t r = 0; r < rows; ++r) {
         sweep_cfg c{shapes[3 * r], shapes[3 * r + 1], shapes[3 * r + 2]};
@@ -750,36 +757,37 @@ int logo2opt_tuning_set(const uint32_t *max_n, const int32_t *shapes, uint32_t r
         v.push_back({max_n[r], c});
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_loaded = true;
-    install_rows_locked(v);
+    g_tune_loaded[table] = true;
+    install_rows_locked(table, v);
     return LOGO2OPT_OK;
 }
 
-int logo2opt_tuning_get(uint32_t *max_n, int32_t *shapes, uint32_t cap, uint32_t *rows)
+int logo2opt_tuning_get(int table, uint32_t *max_n, int32_t *shapes, uint32_t cap, uint32_t *rows)
 {
-    if (!rows) return LOGO2OPT_EINVAL;
+    if (table < 0 || table > 1 || !rows) return LOGO2OPT_EINVAL;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    load_tuning_locked();
-    for (uint32_t r = 0; r < g_tune.size() && r < cap; ++r) {
-        if (max_n) max_n[r] = g_tune[r].max_n;
+    load_tuning_locked(table);
+    const auto &g = g_tune[table];
+    for (uint32_t r = 0; r < g.size() && r < cap; ++r) {
+        if (max_n) max_n[r] = g[r].max_n;
         if (shapes) {
-            shapes[3 * r] = g_tune[r].cfg.k;
-            shapes[3 * r + 1] = g_tune[r].cfg.t;
-            shapes[3 * r + 2] = g_tune[r].cfg.waves;
+            shapes[3 * r] = g[r].cfg.k;
+            shapes[3 * r + 1] = g[r].cfg.t;
+            shapes[3 * r + 2] = g[r].cfg.waves;
         }
     }
-    *rows = (uint32_t)g_tune.size();
+    *rows = (uint32_t)g.size();
     return LOGO2OPT_OK;
 }
 
-int logo2opt_tuning_load(const char *path)
+int logo2opt_tuning_load(int table, const char *path)
 {
-    if (!path) return LOGO2OPT_EINVAL;
+    if (table < 0 || table > 1 || !path) return LOGO2OPT_EINVAL;
     std::vector<tune_row> v;
     if (read_tuning_file(path, v) <= 0) return LOGO2OPT_EINVAL;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_loaded = true;
-    install_rows_locked(v);
+    g_tune_loaded[table] = true;
+    install_rows_locked(table, v);
     return LOGO2OPT_OK;
 }
 
@@ -1107,6 +1115,26 @@ int logo2opt_get_stats(logo2opt_ctx *ctx, logo2opt_stats *st)
     st->sweep_ms = ctx->sweep_ms + ctx->e->sweep_ms;
     st->wall_ms = ctx->wall_ms;
     st->sweep_launches = ctx->e->sweep_launches;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_engine_time_sweep(logo2opt_engine *e, uint32_t sweeps, double *avg_ms)
+{
+    if (!e || !avg_ms || sweeps == 0) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    int rc = rebuild_lists(e);
+    if (rc) return rc;
+    for (const auto &g : e->groups)  // untimed first launch
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+    L2O_TRY(hipEventRecord(e->ev0, e->stream));
+    for (uint32_t s = 0; s < sweeps; ++s)
+        for (const auto &g : e->groups)
+            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+    L2O_TRY(hipEventRecord(e->ev1, e->stream));
+    L2O_TRY(hipEventSynchronize(e->ev1));
+    float ms = 0.0f;
+    L2O_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    *avg_ms = (double)ms / sweeps;
     return LOGO2OPT_OK;
 }
 

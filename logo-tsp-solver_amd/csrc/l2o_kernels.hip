@@ -13,8 +13,8 @@
 // is the first term of step a+1, so each candidate costs ONE new distance
 // (one v_sqrt_f32) instead of two.  A workgroup owns DIAG consecutive
 // diagonals and T consecutive steps; the ordered points it touches are staged
-// once in LDS (rows: uniform per step, a broadcast read; columns: one float4
-// per lane per step, bank-conflict free).
+// once in LDS (rows: uniform per step, a broadcast read; columns: SoA words
+// read per lane without bank conflicts).
 //
 // Exactness.  The FP32 estimate g~ = q(a,b) + q(a+1,b+1) - D[a+1] - D[b+1]
 // differs from the exact integer gain by at most `slack` (per instance, set by
@@ -53,6 +53,20 @@ __device__ __forceinline__ int exact_dist(float x1, float y1, float x2, float y2
     return (int)(k + (s > k * k + k ? 1 : 0));
 }
 
+// FP32 estimate of d(P1,P2): the second filter tier (error bounded by slack)
+__device__ __forceinline__ float est_dist(float x1, float y1, float x2, float y2)
+{
+    const float dx = x1 - x2, dy = y1 - y2;
+    return __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+}
+
+// Chebyshev lower bound max(|dx|, |dy|) <= d(P1,P2): exact in FP32 for integer
+// coordinates below 2^22 (the differences are exact), and no square root
+__device__ __forceinline__ float cheb_dist(float x1, float y1, float x2, float y2)
+{
+    return __builtin_fmaxf(__builtin_fabsf(x1 - x2), __builtin_fabsf(y1 - y2));
+}
+
 __device__ __forceinline__ unsigned long long make_key(int gain, uint32_t i, uint32_t j)
 {
     return ((unsigned long long)(gain + L2O_KEY_GAIN_BIAS) << 40) |
@@ -89,14 +103,24 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 // LDS layout of one workgroup (all float):
 //   rx, ry, rd [T+1]   rows, index 0 = P[a0], index t+1 = P[a0+t+1] (SoA so
 //                      that four consecutive steps are one broadcast read)
-//   col[T + DIAG]      float4 {x, y, D, 0} of P[a0+g0+c]
+//   cx, cy, cd [T+DIAG] columns P[a0+g0+c], SoA: a lane reads 4-byte words at
+//                      a K-word stride, conflict free (an AoS float4 array read
+//                      as ds_read_b96 at a 16-byte stride measured 65% bank-
+//                      conflict cycles and kept LDS busy 86% of the time)
 //   u[WAVES][T]        per-wave thr + D[a0+t+1]: the whole per-step
 //                      threshold comes from LDS, so a candidate costs
 //                      2 sub + mul + fma + sqrt + add + sub + cmp on the VALU
+//   With K = 2 a lane's words sit 2 apart, so lanes l and l+16 would share a
+//   bank: the columns are stored twice, the second copy an odd number of words
+//   later, and lane l reads copy bit4(l) ^ bit5(l).  Every 32 consecutive lanes
+//   then hit 32 distinct banks of 32, and all 64 lanes 64 distinct banks of 64.
 template <int K, int T, int WAVES>
 struct sweep_smem {
+    static constexpr int NC = T + 64 * WAVES * K + 4;  // columns per copy
+    static constexpr int CP = K == 2 ? 2 : 1;         // copies
+    static constexpr int SC = NC + 2;                 // copy stride, even
     float rx[T + 4], ry[T + 4], rd[T + 4];
-    float4 col[T + 64 * WAVES * K + 4];
+    float cx[CP * SC], cy[CP * SC], cd[CP * SC];
     float u[WAVES][T];
     unsigned long long key[WAVES];
     unsigned fb[WAVES];
@@ -120,7 +144,22 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     const int a0 = (int)it.a0, g0 = (int)it.g0;
     const float slack = in->slack;
 
-    for (int c = threadIdx.x; c < NCOL; c += NT) sm.col[c] = P[min(a0 + g0 + c, n - 1)];
+    using SM = sweep_smem<K, T, WAVES>;
+    for (int c = threadIdx.x; c < NCOL; c += NT) {
+        const float4 p = P[min(a0 + g0 + c, n - 1)];
+#pragma unroll
+        for (int q = 0; q < SM::CP; ++q) {
+            sm.cx[q * SM::SC + q + c] = p.x;
+            sm.cy[q * SM::SC + q + c] = p.y;
+            sm.cd[q * SM::SC + q + c] = p.z;
+        }
+    }
+    // this lane's copy of the columns (see sweep_smem)
+    const int coff = SM::CP == 2 ? ((((threadIdx.x >> 4) ^ (threadIdx.x >> 5)) & 1) * (SM::SC + 1)) : 0;
+    const float *__restrict__ ccx = sm.cx + coff;
+    const float *__restrict__ ccy = sm.cy + coff;
+    const float *__restrict__ ccd = sm.cd + coff;
+    auto col = [&](int c) { return make_float4(ccx[c], ccy[c], ccd[c], 0.0f); };
     for (int r = threadIdx.x; r <= T; r += NT) {
         const float4 p = P[min(a0 + r, n - 1)];
         sm.rx[r] = p.x;
@@ -144,37 +183,52 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-    float qp[K];
+    // Two filter tiers.  Tier 1 (every candidate) bounds the gain from below
+    // with Chebyshev distances: nint(d) >= d - 1/2 >= cheb - 1/2, so an exact
+    // gain <= best-1 implies cheb(a,b) + cheb(a+1,b+1) - D[a+1] - D[b+1] <=
+    // best; that sum is an exact FP32 integer and the test against
+    // thr + D[a+1] (thr = best-1+slack) keeps slack-1 of margin for the FP32
+    // rounding of the threshold.  Tier 2 (lanes flagged by tier 1) is the FP32
+    // distance estimate, |exact - estimate| <= slack.  Both tiers only drop
+    // candidates that cannot reach best-1, so the exact pass sees every move
+    // that can win.  A wave ballot of tier 1 is rarely nonzero: the Chebyshev
+    // sum exceeds the two tour edges unless both pairs are spatial neighbours.
+    float qp[K];  // tier-1 bound of the previous step, per diagonal
     {
         const float x0 = sm.rx[0], y0 = sm.ry[0];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float4 C = sm.col[cb + k];
-            const float dx = x0 - C.x, dy = y0 - C.y;
-            qp[k] = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+            const float4 C = col(cb + k);
+            qp[k] = cheb_dist(x0, y0, C.x, C.y);
         }
     }
     unsigned long long lbest = L2O_SENTINEL;
     unsigned fb = 0;
 
-    // exact re-evaluation of flagged candidates (steps t0 .. t0+ns-1); then
-    // the wave's threshold and its per-step thresholds in LDS are refreshed
-    auto slow = [&](int t0, int ns, const bool *hit) {
+    // tier 2 and exact re-evaluation of tier-1 hits (steps t0 .. t0+ns-1);
+    // then the wave's threshold and its per-step thresholds in LDS are
+    // refreshed.  Hit masks are wave ballots (SGPR pairs): lane bits are only
+    // extracted here.
+    auto slow = [&](int t0, int ns, const unsigned long long *hit) {
         for (int s = 0; s < ns; ++s) {
             const int t = t0 + s;
             const float xa = sm.rx[t], ya = sm.ry[t], xb = sm.rx[t + 1], yb = sm.ry[t + 1], db = sm.rd[t + 1];
+            const float u = su[t];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                if (hit[s * K + k]) {
-                    ++fb;
-                    const float4 Ca = sm.col[t + cb + k], Cb = sm.col[t + 1 + cb + k];
-                    const int gain = exact_dist(xa, ya, Ca.x, Ca.y) + exact_dist(xb, yb, Cb.x, Cb.y) - (int)db -
-                                     (int)Cb.z;
-                    if (gain < 0) {
-                        const uint32_t i = (uint32_t)(a0 + t + 1);
-                        const uint32_t j = (uint32_t)(a0 + t + 1 + g0 + cb + k);
-                        const unsigned long long key = make_key(gain, i, j);
-                        lbest = key < lbest ? key : lbest;
+                if ((hit[s * K + k] >> lane) & 1ull) {
+                    const float4 Ca = col(t + cb + k), Cb = col(t + 1 + cb + k);
+                    const float w2 = (est_dist(xa, ya, Ca.x, Ca.y) + est_dist(xb, yb, Cb.x, Cb.y)) - Cb.z;
+                    if (w2 <= u) {
+                        ++fb;
+                        const int gain = exact_dist(xa, ya, Ca.x, Ca.y) + exact_dist(xb, yb, Cb.x, Cb.y) -
+                                         (int)db - (int)Cb.z;
+                        if (gain < 0) {
+                            const uint32_t i = (uint32_t)(a0 + t + 1);
+                            const uint32_t j = (uint32_t)(a0 + t + 1 + g0 + cb + k);
+                            const unsigned long long key = make_key(gain, i, j);
+                            lbest = key < lbest ? key : lbest;
+                        }
                     }
                 }
             }
@@ -193,53 +247,61 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         }
     };
 
+    // Main loop.  Row points are wave-uniform: they come from global memory
+    // through the scalar cache (below t_full every row index a0+t+1+s is
+    // < n).  Column words are read through volatile pointers so that each is
+    // one ds_read_b32 off a single per-lane base with an immediate offset:
+    // merged ds_read2 pairs would need a fresh base VGPR per array (their
+    // 8-bit offsets cannot span the column arrays), one VALU add each.
+    const __attribute__((address_space(4))) float *__restrict__ prow =
+        (const __attribute__((address_space(4))) float *)(P + a0 + 1);  // row t at prow[4t]
+    typedef const volatile __attribute__((address_space(3))) float lds_vf;
+    lds_vf *vcx = (lds_vf *)(ccx + cb + 1), *vcy = (lds_vf *)(ccy + cb + 1), *vcd = (lds_vf *)(ccd + cb + 1);
     int t = 0;
     const int t_main = t_full - t_full % UF;
     for (; t < t_main; t += UF) {
         float x[UF], y[UF], U[UF];
 #pragma unroll
         for (int s = 0; s < UF; ++s) {
-            x[s] = sm.rx[t + 1 + s];
-            y[s] = sm.ry[t + 1 + s];
+            x[s] = prow[4 * (t + s)];
+            y[s] = prow[4 * (t + s) + 1];
             U[s] = su[t + s];
         }
         float4 C[UF + K - 1];
 #pragma unroll
-        for (int j = 0; j < UF + K - 1; ++j) C[j] = sm.col[t + 1 + cb + j];
-        bool hit[UF * K];
-        bool any = false;
+        for (int j = 0; j < UF + K - 1; ++j) C[j] = make_float4(vcx[t + j], vcy[t + j], vcd[t + j], 0.0f);
+        unsigned long long hit[UF * K];
+        unsigned long long any = 0;
 #pragma unroll
         for (int s = 0; s < UF; ++s) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const float dx = x[s] - C[s + k].x, dy = y[s] - C[s + k].y;
-                const float q = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+                const float q = cheb_dist(x[s], y[s], C[s + k].x, C[s + k].y);
                 const float w2 = (qp[k] + q) - C[s + k].z;
                 qp[k] = q;
-                hit[s * K + k] = w2 <= U[s];
+                hit[s * K + k] = __ballot(w2 <= U[s]);
                 any |= hit[s * K + k];
             }
         }
-        if (__ballot(any)) slow(t, UF, hit);
+        if (any) slow(t, UF, hit);
     }
     int lim[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) lim[k] = n - 2 - a0 - (g0 + cb + k);
     for (; t < t_end; ++t) {
         const float x = sm.rx[t + 1], y = sm.ry[t + 1], U = su[t];
-        bool hit[K];
-        bool any = false;
+        unsigned long long hit[K];
+        unsigned long long any = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float4 C = sm.col[t + 1 + cb + k];
-            const float dx = x - C.x, dy = y - C.y;
-            const float q = __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+            const float4 C = col(t + 1 + cb + k);
+            const float q = cheb_dist(x, y, C.x, C.y);
             const float w2 = (qp[k] + q) - C.z;
             qp[k] = q;
-            hit[k] = (w2 <= U) && (t <= lim[k]);
+            hit[k] = __ballot((w2 <= U) && (t <= lim[k]));
             any |= hit[k];
         }
-        if (__ballot(any)) slow(t, 1, hit);
+        if (any) slow(t, 1, hit);
     }
 
     const unsigned long long wb = wave_min_u64(lbest);

@@ -50,6 +50,13 @@ struct request {
 struct logo2opt_server {
     logo2opt_engine *e = nullptr;
     uint32_t queue_cap = 0;
+    // admission limits beyond HBM and slots: running requests, and the sum
+    // of candidate moves per sweep over running requests (0 = no limit).  A
+    // saturated GPU gains nothing from more concurrent requests; they only
+    // stretch every request's latency (processor sharing).
+    uint32_t max_running = 0;
+    uint64_t max_inflight_pairs = 0;
+    uint64_t inflight_pairs = 0;
     int64_t next_id = 1;
     std::deque<int64_t> queue;
     std::map<int64_t, request> reqs;
@@ -114,6 +121,10 @@ int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every
     if (!s) return LOGO2OPT_EINVAL;
     while (!s->queue.empty()) {
         request &r = s->reqs[s->queue.front()];
+        const uint64_t pairs = r.n >= 4 ? (uint64_t)(r.n - 2) * (r.n - 3) / 2 : 0;
+        if (s->max_running && s->slot_req.size() >= s->max_running) break;
+        if (s->max_inflight_pairs && !s->slot_req.empty() && s->inflight_pairs + pairs > s->max_inflight_pairs)
+            break;
         int32_t slot = -1;
         const int rc = logo2opt_engine_add(s->e, r.n, r.coords.data(), r.route.data(), &r.job, &slot);
         if (rc == LOGO2OPT_EAGAIN) break;  // head of line does not fit yet
@@ -129,6 +140,7 @@ int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every
         r.t_admit = now_s();
         s->slot_req[slot] = r.id;
         s->n_admitted++;
+        s->inflight_pairs += pairs;
         // host copies are no longer needed: the instance is resident in HBM
         std::vector<logo2opt_city_coords>().swap(r.coords);
     }
@@ -155,6 +167,7 @@ int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every
             r.state = Q_DONE;
             r.t_done = t;
             r.slot = -1;
+            s->inflight_pairs -= r.n >= 4 ? (uint64_t)(r.n - 2) * (r.n - 3) / 2 : 0;
             s->slot_req.erase(it);
             s->completed.push_back(r.id);
             s->n_done++;
@@ -199,10 +212,20 @@ int logo2opt_server_release(logo2opt_server *s, int64_t id)
         const int rc = logo2opt_engine_remove(s->e, it->second.slot);
         if (rc) return rc;
         s->slot_req.erase(it->second.slot);
+        const uint32_t n = it->second.n;
+        s->inflight_pairs -= n >= 4 ? (uint64_t)(n - 2) * (n - 3) / 2 : 0;
     } else if (it->second.state == Q_QUEUED) {
         s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
     }
     s->reqs.erase(it);
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_t max_inflight_pairs)
+{
+    if (!s) return LOGO2OPT_EINVAL;
+    s->max_running = max_running;
+    s->max_inflight_pairs = max_inflight_pairs;
     return LOGO2OPT_OK;
 }
 

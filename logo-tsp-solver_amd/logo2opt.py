@@ -25,6 +25,7 @@ EXPORTS = [
     "logo2opt_engine_poll_done", "logo2opt_engine_stats", "logo2opt_engine_set_timing", "logo2opt_engine_set_graphs", "logo2opt_gen_uniform", "logo2opt_gen_clustered",
     "logo2opt_random_perm", "logo2opt_nn_tour", "logo2opt_xy_to_coords", "logo2opt_sqrt_ulp_check",
     "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
+    "logo2opt_engine_time_sweep",
 ]
 
 
@@ -115,9 +116,10 @@ def load():
     L.logo2opt_xy_to_coords.restype = None
     L.logo2opt_sqrt_ulp_check.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
     L.logo2opt_tuning_shapes.argtypes = [i32p, C.c_uint32, C.POINTER(C.c_uint32)]
-    L.logo2opt_tuning_set.argtypes = [u32p, i32p, C.c_uint32]
-    L.logo2opt_tuning_get.argtypes = [u32p, i32p, C.c_uint32, C.POINTER(C.c_uint32)]
-    L.logo2opt_tuning_load.argtypes = [C.c_char_p]
+    L.logo2opt_tuning_set.argtypes = [C.c_int, u32p, i32p, C.c_uint32]
+    L.logo2opt_tuning_get.argtypes = [C.c_int, u32p, i32p, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.logo2opt_tuning_load.argtypes = [C.c_int, C.c_char_p]
+    L.logo2opt_engine_time_sweep.argtypes = [vp, C.c_uint32, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -191,23 +193,27 @@ def tuning_shapes():
     return [tuple(int(v) for v in out[3 * q:3 * q + 3]) for q in range(cnt.value)]
 
 
-def tuning_set(rows):
+TABLE_SINGLE = 0  # fewer than 8 live instances in an engine
+TABLE_BATCH = 1
+
+
+def tuning_set(rows, table=TABLE_SINGLE):
     """rows: [(max_n, (k, t, waves)), ...]"""
     mx = np.array([r[0] for r in rows], np.uint32)
     sh = np.array([v for r in rows for v in r[1]], np.int32)
-    _check(load().logo2opt_tuning_set(mx, sh, len(rows)), "tuning_set")
+    _check(load().logo2opt_tuning_set(table, mx, sh, len(rows)), "tuning_set")
 
 
-def tuning_get():
+def tuning_get(table=TABLE_SINGLE):
     mx = np.zeros(64, np.uint32)
     sh = np.zeros(3 * 64, np.int32)
     cnt = C.c_uint32()
-    _check(load().logo2opt_tuning_get(mx, sh, 64, C.byref(cnt)), "tuning_get")
+    _check(load().logo2opt_tuning_get(table, mx, sh, 64, C.byref(cnt)), "tuning_get")
     return [(int(mx[q]), tuple(int(v) for v in sh[3 * q:3 * q + 3])) for q in range(cnt.value)]
 
 
-def tuning_load(path):
-    _check(load().logo2opt_tuning_load(path.encode()), "tuning_load")
+def tuning_load(path, table=TABLE_SINGLE):
+    _check(load().logo2opt_tuning_load(table, path.encode()), "tuning_load")
 
 
 # ---- single-instance context (the CUDASolver replacement) -----------------
@@ -350,6 +356,11 @@ class Engine:
             out["log"] = np.array([(log[q].i, log[q].j, log[q].minchange) for q in range(k)], np.int64).reshape(k, 3)
         return out
 
+    def time_sweep(self, sweeps):
+        ms = C.c_double()
+        _check(load().logo2opt_engine_time_sweep(self.h, sweeps, C.byref(ms)), "engine_time_sweep")
+        return ms.value
+
     def round_lengths(self, slot, cap):
         out = np.zeros(cap, np.uint64)
         _check(load().logo2opt_engine_round_lengths(self.h, slot, out, cap), "round_lengths")
@@ -379,7 +390,8 @@ class Engine:
 # ---- server (continuous batching of requests) -------------------------------
 SERVER_EXPORTS = ["logo2opt_server_create", "logo2opt_server_destroy", "logo2opt_server_submit",
                   "logo2opt_server_pump", "logo2opt_server_poll", "logo2opt_server_result",
-                  "logo2opt_server_release", "logo2opt_server_counts", "logo2opt_server_replay"]
+                  "logo2opt_server_release", "logo2opt_server_counts", "logo2opt_server_replay",
+                  "logo2opt_server_set_limits"]
 EXPORTS += SERVER_EXPORTS
 
 
@@ -395,6 +407,7 @@ def _bind_server(L):
                                          C.POINTER(C.c_double)]
     L.logo2opt_server_release.argtypes = [vp, C.c_int64]
     L.logo2opt_server_counts.argtypes = [vp, u64p]
+    L.logo2opt_server_set_limits.argtypes = [vp, C.c_uint32, C.c_uint64]
     L.logo2opt_server_replay.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_double, C.c_uint32, C.c_uint32,
                                          vp, vp, C.POINTER(C.c_double)]
     L._server_bound = True
@@ -452,6 +465,9 @@ class Server:
         out = {f: getattr(res, f) for f, _ in Result._fields_}
         out.update(route=r, latency_s=lat.value, queue_s=qs.value)
         return out
+
+    def set_limits(self, max_running=0, max_inflight_pairs=0):
+        _check(load().logo2opt_server_set_limits(self.h, max_running, max_inflight_pairs), "server_set_limits")
 
     def counts(self):
         c = np.zeros(6, np.uint64)

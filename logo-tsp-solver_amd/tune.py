@@ -1,14 +1,19 @@
 """Autotuner for the sweep kernel's tile shape, per n bucket.
 
-    python logo-tsp-solver_amd/tune.py [--out logo-tsp-solver_amd/tuning_gfx950.txt]
+    python logo-tsp-solver_amd/tune.py [--out-dir logo-tsp-solver_amd]
 
-For each representative city count it times every compiled shape
-{k diagonals per lane, t steps, waves per workgroup} with HIP events
-(logo2opt_time_sweep) on two tour states -- a random permutation and a greedy
-nearest-neighbour tour -- and keeps the fastest.  Bucket edges are the
-geometric midpoints between representatives.  The result is a text table
-("max_n k t waves" per line) that the library loads on first use from the
-file next to liblogo2opt.so (or from $LOGO2OPT_TUNING).
+Writes two tables ("max_n k t waves" per line) that the library loads on first
+use from its own directory:
+  tuning_gfx950.txt        one instance in flight (logo2opt_create contexts,
+                           engines with fewer than 8 live instances): each
+                           shape is timed with HIP events (logo2opt_time_sweep)
+                           on a random and a nearest-neighbour tour
+  tuning_batch_gfx950.txt  batches (8+ live instances): each shape is timed on
+                           an engine holding enough instances of that size to
+                           fill the GPU (logo2opt_engine_time_sweep), and the
+                           time per candidate move decides
+A shape is {k diagonals per lane, t steps, waves per workgroup}.  Bucket edges
+are the geometric midpoints between the representative sizes.
 """
 import argparse
 import math
@@ -20,17 +25,15 @@ sys.path.insert(0, HERE)
 
 import logo2opt as L  # noqa: E402
 
-DEFAULT_OUT = os.path.join(HERE, "tuning_gfx950.txt")
 REPS = (100, 250, 500, 1000, 2000, 4000, 8000, 16000, 33810, 100000)
+BATCH_REPS = (100, 250, 500, 1000, 2000, 4000, 8000, 16000)
 
 
-def time_shape(ctx, tours, shape, sweeps):
-    L.tuning_set([(L.MAX_N, shape)])
-    return sum(ctx.time_sweep(t, sweeps) for t in tours)
+def pairs(n):
+    return (n - 2) * (n - 3) // 2
 
 
-def tune(reps=REPS, seed=1, verbose=True):
-    shapes = L.tuning_shapes()
+def tune_single(reps=REPS, seed=1, verbose=True):
     best = []
     for n in reps:
         side = 10**4 if n <= 5000 else 10**6
@@ -39,23 +42,52 @@ def tune(reps=REPS, seed=1, verbose=True):
         ctx = L.Context(L.coords_of(xy))
         sweeps = max(3, min(50, int(2e9 / (n * n))))
         times = {}
-        for sh in shapes:
-            time_shape(ctx, tours, sh, 1)  # first launch of this shape
-            times[sh] = time_shape(ctx, tours, sh, sweeps) / len(tours)
+        for sh in L.tuning_shapes():
+            L.tuning_set([(L.MAX_N, sh)], L.TABLE_SINGLE)
+            for t in tours:
+                ctx.time_sweep(t, 1)  # first launch of this shape
+            times[sh] = sum(ctx.time_sweep(t, sweeps) for t in tours) / len(tours)
         ctx.close()
         b = min(times, key=times.get)
-        best.append((n, b, times[b]))
+        best.append((n, b))
         if verbose:
             ranked = sorted(times.items(), key=lambda kv: kv[1])[:4]
-            print(f"n={n:>6}: " + "  ".join(f"{s}:{ms * 1e3:.1f}us" for s, ms in ranked), flush=True)
+            print(f"single n={n:>6}: " + "  ".join(f"{s}:{ms * 1e3:.1f}us" for s, ms in ranked), flush=True)
+    return best
+
+
+def tune_batch(reps=BATCH_REPS, seed=2, verbose=True):
+    best = []
+    for n in reps:
+        m = max(8, min(1024, int(4e8 / pairs(n))))  # ~4e8 candidates per batched sweep
+        eng = L.Engine(hbm_budget=m * L.Engine.bytes_for(n) + (16 << 20), max_slots=m)
+        for q in range(m):
+            xy = L.gen_uniform(n, 10**4, seed * 100000 + q)
+            eng.add(L.coords_of(xy), L.random_perm(n, seed + q))
+        times = {}
+        for sh in L.tuning_shapes():
+            L.tuning_set([(L.MAX_N, sh)], L.TABLE_BATCH)
+            eng.time_sweep(1)
+            times[sh] = eng.time_sweep(5) / (m * pairs(n))
+        eng.close()
+        b = min(times, key=times.get)
+        best.append((n, b))
+        if verbose:
+            ranked = sorted(times.items(), key=lambda kv: kv[1])[:4]
+            print(f"batch  n={n:>6} (x{m}): " + "  ".join(f"{s}:{1e9 * ms * 1e-3:.3f}ns/eval" for s, ms in ranked),
+                  flush=True)
+    return best
+
+
+def table_of(best):
     rows = []
-    for q, (n, sh, _) in enumerate(best):
+    for q, (n, sh) in enumerate(best):
         edge = L.MAX_N if q == len(best) - 1 else int(math.sqrt(n * best[q + 1][0]))
         if rows and rows[-1][1] == sh:
             rows[-1] = (edge, sh)
         else:
             rows.append((edge, sh))
-    return rows, best
+    return rows
 
 
 def write_table(rows, path):
@@ -66,12 +98,17 @@ def write_table(rows, path):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--out-dir", default=HERE)
     args = ap.parse_args()
-    rows, best = tune()
-    write_table(rows, args.out)
-    print("table:", rows)
-    print("wrote", args.out)
+    single_best = tune_single()
+    single = table_of(single_best)
+    # above the largest batch representative one instance fills the GPU on its
+    # own, so the batch table continues with the single-instance choices
+    batch = table_of(tune_batch() + [(n, sh) for n, sh in single_best if n > BATCH_REPS[-1]])
+    write_table(single, os.path.join(args.out_dir, "tuning_gfx950.txt"))
+    write_table(batch, os.path.join(args.out_dir, "tuning_batch_gfx950.txt"))
+    print("single:", single)
+    print("batch:", batch)
 
 
 if __name__ == "__main__":
