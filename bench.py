@@ -37,7 +37,10 @@ sys.path.insert(0, os.path.join(ROOT, "logo-tsp-solver_amd"))
 import logo2opt as L  # noqa: E402
 
 METRIC = "2-opt move evals/sec + instances/sec batched vs plain, bit-exact tour length"
-FLOP_PER_EVAL = 14        # 2 new edges x (2 sub + mul + fma(2) + sqrt + add)
+# algorithmic work of one move evaluation, the unit the roofline is priced in:
+# 2 new edges x (2 sub + mul + fma(2) + sqrt + add); the kernel's own filter
+# does less (integer L1 bounds, see l2o_kernels.hip) and is priced the same
+FLOP_PER_EVAL = 14
 REF_FLOP_PER_EVAL = 36    # the reference's GPU GFLOPS convention (solver.c:305-306)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 
@@ -48,6 +51,13 @@ def log(*a):
 
 def pairs(n):
     return (n - 2) * (n - 3) // 2
+
+
+def sweep_kernel_name(n, table=None):
+    """Name of the sweep kernel instance the library launches for size n."""
+    rows = L.tuning_get(L.TABLE_SINGLE if table is None else table)
+    k, t, w = next((sh for mx, sh in rows if n <= mx), rows[-1][1])
+    return f"l2o::k_sweep<{k},{t},{w}>"
 
 
 def config2(seed=2):
@@ -272,7 +282,7 @@ def main():
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                "kernel": "l2o::k_sweep<2,256,4>", "avg_kernel_ms": avg_ms, "launches": launches,
+                "kernel": sweep_kernel_name(n), "avg_kernel_ms": avg_ms, "launches": launches,
                 "evals_per_s_kernel": pairs(n) * work_sweeps / launches / (avg_ms * 1e-3),
                 "ref_convention_tflops": achieved * REF_FLOP_PER_EVAL / FLOP_PER_EVAL}
     ctx.close()

@@ -61,8 +61,8 @@ struct tune_row {
 #define L2O_BATCH_MIN 8
 std::mutex g_tune_mu;
 std::vector<tune_row> g_tune[2] = {
-    {{640, {1, 64, 1}}, {8192, {1, 128, 4}}, {LOGO2OPT_MAX_N, {2, 256, 4}}},
-    {{640, {1, 64, 1}}, {8192, {1, 128, 4}}, {LOGO2OPT_MAX_N, {2, 256, 4}}},
+    {{640, {1, 64, 1}}, {8192, {4, 128, 1}}, {LOGO2OPT_MAX_N, {8, 256, 1}}},
+    {{640, {1, 64, 1}}, {8192, {4, 128, 1}}, {LOGO2OPT_MAX_N, {8, 256, 1}}},
 };
 bool g_tune_loaded[2] = {false, false};
 const char *g_tune_file[2] = {"tuning_gfx950.txt", "tuning_batch_gfx950.txt"};
@@ -331,13 +331,13 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
     char *base = e->d_arena + s.off;
     l2o_inst in;
     std::memset(&in, 0, sizeof(in));
-    in.pts[0] = (float4 *)base;
-    in.pts[1] = (float4 *)(base + al(16ull * n));
+    in.pts[0] = (l2o_pt *)base;
+    in.pts[1] = (l2o_pt *)(base + al(16ull * n));
     in.tour[0] = (uint32_t *)(base + 2 * al(16ull * n));
     in.tour[1] = (uint32_t *)(base + 2 * al(16ull * n) + al(4ull * n));
     char *p = base + 2 * al(16ull * n) + 2 * al(4ull * n);
     if (s.job.ils_rounds) {
-        in.bpts = (float4 *)p;
+        in.bpts = (l2o_pt *)p;
         p += al(16ull * n);
         in.btour = (uint32_t *)p;
         p += al(4ull * n);
@@ -360,14 +360,17 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         in.rng[k] = z ^ (z >> 31);
     }
-    std::vector<float4> pts(n);
+    std::vector<l2o_pt> pts(n);
     uint64_t L = 0;
     const int32_t *xy = s.xy.data();
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t c = route[k], pc = route[k == 0 ? n - 1 : k - 1];
         const uint32_t d = dist_i(xy, pc, c);
         L += d;
-        pts[k] = make_float4((float)xy[2 * c], (float)xy[2 * c + 1], (float)d, 0.0f);
+        pts[k].x = xy[2 * c];
+        pts[k].y = xy[2 * c + 1];
+        pts[k].d = (int32_t)d;
+        pts[k].w = l2o::ceil_sqrt2((int32_t)d);
     }
     in.length = L;
     L2O_TRY(hipMemcpyAsync(in.pts[0], pts.data(), 16ull * n, hipMemcpyHostToDevice, e->stream));
@@ -405,10 +408,10 @@ int rebuild_lists(logo2opt_engine *e)
         const uint32_t n = s.n;
         const sweep_cfg c = cfg_for(n, table);
         auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves)];
-        for (uint32_t g0 = 2; g0 + 2 <= n; g0 += c.diag()) {
-            const uint32_t len = n - 1 - g0;
-            for (uint32_t a0 = 0; a0 < len; a0 += c.t) v.push_back({id, g0, a0, 0});
-        }
+        // rows a <= n-4; the first tile of a row block starts at the
+        // diagonal, b0 = a0 + 2
+        for (uint32_t a0 = 0; a0 + 4 <= n; a0 += c.t)
+            for (uint32_t b0 = a0 + 2; b0 + 2 <= n; b0 += c.cols()) v.push_back({id, a0, b0, 0});
         const uint32_t nch = (n + L2O_APPLY_CHUNK - 1) / L2O_APPLY_CHUNK;
         for (uint32_t ch = 0; ch < nch; ++ch) ai.push_back({id, ch});
         if (s.job.ils_rounds) {
@@ -420,10 +423,9 @@ int rebuild_lists(logo2opt_engine *e)
     e->groups.clear();
     for (auto &kv : by_cfg) {
         const sweep_cfg c{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
-        // Persistent workgroups take items in list order from a ticket
-        // counter, so at any moment the whole device works on a window of
-        // consecutive items of one or a few instances (largest first), whose
-        // points stay hot in every XCD's L2.
+        // Items are dispatched in list order (largest instances first), so at
+        // any moment the device works on a window of consecutive items of one
+        // or a few instances, whose points stay hot in every XCD's L2.
         const auto &v = kv.second;
         const uint32_t cnt = (uint32_t)v.size();
         if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;

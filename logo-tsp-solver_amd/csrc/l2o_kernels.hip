@@ -3,37 +3,47 @@
 // Replaces the reference's kernel2opt / kernel2opt_extended
 // (src/cuda/opt_kernel.cu:34-262, 273-627) and the host-side apply
 // (src/solvers/twoOptLocalSolver.cpp:40-50).  Wave64, no MFMA: the sweep is
-// FP32 VALU + transcendental work.
+// integer VALU work (v_sad_u32) with rare FP32/FP64 re-evaluation.
 //
-// Sweep geometry.  With a = i-1 and b = J-1 a candidate is a pair
-// 0 <= a, a+2 <= b <= n-2 with gain
-//     M(a,b) + M(a+1,b+1) - D[a+1] - D[b+1],   M(x,y) = d(P[x],P[y]),
-// D[k] = d(P[k-1],P[k])  (cpuSolver.cpp:133-138 in tour-position form).
-// A lane walks diagonals g = b-a: the distance M(a+1,b+1) computed at step a
-// is the first term of step a+1, so each candidate costs ONE new distance
-// (one v_sqrt_f32) instead of two.  A workgroup owns DIAG consecutive
-// diagonals and T consecutive steps; the ordered points it touches are staged
-// once in LDS (rows: uniform per step, a broadcast read; columns: SoA words
-// read per lane without bank conflicts).
+// Candidates.  With a = i-1 and b = J-1 a candidate is a pair
+// 0 <= a, a+2 <= b <= n-2 with exact gain
+//     G = n(a,b) + n(a+1,b+1) - D[a+1] - D[b+1],   n(x,y) = nint(d(P[x],P[y])),
+// D[k] = n(k-1,k)  (cpuSolver.cpp:133-138 in tour-position form).  The sweep
+// returns the lexicographic minimum of (G, i, J) over G < 0, the move the
+// reference's strict-'<' scan keeps.
 //
-// Exactness.  The FP32 estimate g~ = q(a,b) + q(a+1,b+1) - D[a+1] - D[b+1]
-// differs from the exact integer gain by at most `slack` (per instance, set by
-// the host from the coordinate span: 1 for the two nint roundings plus the
-// FP32 error of s, of v_sqrt_f32 and of the three adds).  A wave keeps the
-// best exact key it has seen; a candidate is re-evaluated exactly (FP64 sqrt
-// + integer fix-up) only if g~ <= best_gain - 1 + slack, i.e. only if it
-// could still beat that key.  Candidates of later steps have larger i, so a
-// tie with the current best can never win; candidates of the same step are
-// all re-evaluated together.  The wave's result is therefore the exact
-// lexicographic minimum of (gain, i, J) over its candidates, and the global
-// atomicMin over 64-bit keys makes the grid result exact and deterministic
-// (the reference's grid tie-break is a racy lock, opt_kernel.cu:239-261).
+// Three filter tiers make every candidate cost four VALU operations:
+//  1. Integer L1 bound.  nint(d) >= d - 1/2 and |dx|+|dy| <= sqrt(2) d, so
+//     G <= best-1 implies
+//         L1(a,b) + L1(a+1,b+1) <= ceil(sqrt2*best) + W[a+1] + W[b+1],
+//     W[k] = ceil(sqrt(2) D[k]) (stored with each point).  All terms are
+//     integers: the test is exact, no rounding margin.  A lane owns K+1
+//     consecutive columns b and keeps per-column offsets o with
+//     o_k + o_{k+1} = -W[b_k + 1]; then V(a,k) = L1(a, b_k) + o_k is two
+//     v_sad_u32 (|x-x'| + acc) and the test of candidate (a, b_k) is
+//     V(a,k) + V(a+1,k+1) <= ceil(sqrt2*best) + W[a+1], whose right side is
+//     wave-uniform.  Each V is computed once and used by two candidates.
+//  2. FP32 estimate q(a,b) + q(a+1,b+1) - D[a+1] - D[b+1] <= best-1+slack,
+//     |exact - estimate| <= slack (per instance, set by the host from the
+//     coordinate span), only for lanes flagged by tier 1.
+//  3. Exact FP64 sqrt + integer fix-up for the survivors.
+// Each tier only drops candidates with G > cap, where cap is the smaller of
+// (the wave's best exact gain of EARLIER steps) - 1 -- a later step has a
+// larger i, so a tie with that best can never win -- and the best gain
+// published so far by any wave of the sweep (any i, so taken inclusively);
+// candidates of the steps being processed are re-evaluated together.  Every
+// candidate that can still be the lexicographic minimum therefore reaches the
+// exact pass, and the atomicMin over 64-bit keys makes the grid result exact
+// and deterministic whatever the order of waves (the reference's grid
+// tie-break is a racy lock, opt_kernel.cu:239-261).
 #include <hip/hip_runtime.h>
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
+#include "l2o_launch.h"
 #include "l2o_state.h"
 
 namespace l2o {
@@ -41,11 +51,11 @@ namespace l2o {
 // ---------------------------------------------------------------------------
 // exact TSPLIB EUC_2D distance: nint(sqrt(s)) = k + [s > k^2 + k], k = isqrt(s)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int exact_dist(float x1, float y1, float x2, float y2)
+__device__ __forceinline__ int exact_dist(int x1, int y1, int x2, int y2)
 {
-    // coordinates are integers below 2^22: int32 conversion is exact
-    const long long dx = (long long)((int)x1 - (int)x2);
-    const long long dy = (long long)((int)y1 - (int)y2);
+    // coordinates lie in [0, 2^22): the differences and s are exact
+    const long long dx = (long long)(x1 - x2);
+    const long long dy = (long long)(y1 - y2);
     const long long s = dx * dx + dy * dy;
     long long k = (long long)sqrt((double)s);
     if (k * k > s) k -= 1;
@@ -53,18 +63,24 @@ __device__ __forceinline__ int exact_dist(float x1, float y1, float x2, float y2
     return (int)(k + (s > k * k + k ? 1 : 0));
 }
 
-// FP32 estimate of d(P1,P2): the second filter tier (error bounded by slack)
-__device__ __forceinline__ float est_dist(float x1, float y1, float x2, float y2)
+// FP32 estimate of d(P1,P2): the second filter tier (error bounded by slack).
+// The integer differences convert to FP32 exactly.
+__device__ __forceinline__ float est_dist(int x1, int y1, int x2, int y2)
 {
-    const float dx = x1 - x2, dy = y1 - y2;
+    const float dx = (float)(x1 - x2), dy = (float)(y1 - y2);
     return __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
 }
 
-// Chebyshev lower bound max(|dx|, |dy|) <= d(P1,P2): exact in FP32 for integer
-// coordinates below 2^22 (the differences are exact), and no square root
-__device__ __forceinline__ float cheb_dist(float x1, float y1, float x2, float y2)
+// |a - b| + c: one v_sad_u32
+__device__ __forceinline__ uint32_t sad(uint32_t a, uint32_t b, uint32_t c)
 {
-    return __builtin_fmaxf(__builtin_fabsf(x1 - x2), __builtin_fabsf(y1 - y2));
+    return (a > b ? a - b : b - a) + c;
+}
+
+// ceil(sqrt(2) * best) for best <= 0
+__device__ __forceinline__ int tier1_base(int best)
+{
+    return best == 0 ? 0 : 1 - ceil_sqrt2(-best);
 }
 
 __device__ __forceinline__ unsigned long long make_key(int gain, uint32_t i, uint32_t j)
@@ -90,6 +106,15 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
+// a wave-uniform 64-bit value moved to SGPRs (readfirstlane returns int: each
+// half is taken as unsigned before it is widened)
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 {
 #pragma unroll
@@ -100,132 +125,121 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 // ---------------------------------------------------------------------------
 // sweep
 // ---------------------------------------------------------------------------
-// LDS layout of one workgroup (all float):
-//   rx, ry, rd [T+1]   rows, index 0 = P[a0], index t+1 = P[a0+t+1] (SoA so
-//                      that four consecutive steps are one broadcast read)
-//   cx, cy, cd [T+DIAG] columns P[a0+g0+c], SoA: a lane reads 4-byte words at
-//                      a K-word stride, conflict free (an AoS float4 array read
-//                      as ds_read_b96 at a 16-byte stride measured 65% bank-
-//                      conflict cycles and kept LDS busy 86% of the time)
-//   u[WAVES][T]        per-wave thr + D[a0+t+1]: the whole per-step
-//                      threshold comes from LDS, so a candidate costs
-//                      2 sub + mul + fma + sqrt + add + sub + cmp on the VALU
-//   With K = 2 a lane's words sit 2 apart, so lanes l and l+16 would share a
-//   bank: the columns are stored twice, the second copy an odd number of words
-//   later, and lane l reads copy bit4(l) ^ bit5(l).  Every 32 consecutive lanes
-//   then hit 32 distinct banks of 32, and all 64 lanes 64 distinct banks of 64.
-template <int K, int T, int WAVES>
-struct sweep_smem {
-    static constexpr int NC = T + 64 * WAVES * K + 4;  // columns per copy
-    static constexpr int CP = K == 2 ? 2 : 1;         // copies
-    static constexpr int SC = NC + 2;                 // copy stride, even
-    float rx[T + 4], ry[T + 4], rd[T + 4];
-    float cx[CP * SC], cy[CP * SC], cd[CP * SC];
-    float u[WAVES][T];
-    unsigned long long key[WAVES];
-    unsigned fb[WAVES];
-};
+// A work item covers rows a in [a0, a0+T) and columns b in [b0, b0+64*WAVES*K).
+// Lane l of wave w owns the K columns [bl, bl+K), bl = b0 + (64w + l) K, and
+// keeps the K+1 points P[bl .. bl+K] in registers; rows stream through the
+// scalar cache (s_load, uniform addresses), so the loop reads no LDS and no
+// vector memory.  Per step and lane: K+1 values V (2(K+1) v_sad_u32), K adds,
+// a min tree (v_min3) and one compare, ballotted into an SGPR mask.  Waves are
+// independent (no LDS, no barrier): each reduces its own key.
+//
+// Invalid candidates never pass tier 1: a column b > n-2 gets +2^30 in its
+// offset pair, and in a wave that meets the diagonal (some b < a+2) each
+// candidate is masked by its own step limit.
+#define L2O_T1_BIG 0x40000000u
 
-// one work item: all candidates of diagonals [g0, g0+DIAG) x steps [a0, a0+T)
 template <int K, int T, int WAVES>
-__device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__restrict__ insts,
-                                           sweep_smem<K, T, WAVES> &sm)
+__global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__restrict__ items,
+                                                      l2o_inst *__restrict__ insts)
 {
-    constexpr int NT = 64 * WAVES;
-    constexpr int DIAG = NT * K;
-    constexpr int NCOL = T + DIAG;
     constexpr int UF = 4;  // steps per unrolled iteration
     static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
-
+    const l2o_sweep_item it = items[blockIdx.x];
     l2o_inst *in = insts + it.inst;
     if (in->phase != L2O_LO) return;
     const int n = (int)in->n;
-    const float4 *__restrict__ P = in->pts[in->parity];
-    const int a0 = (int)it.a0, g0 = (int)it.g0;
-    const float slack = in->slack;
-
-    using SM = sweep_smem<K, T, WAVES>;
-    for (int c = threadIdx.x; c < NCOL; c += NT) {
-        const float4 p = P[min(a0 + g0 + c, n - 1)];
-#pragma unroll
-        for (int q = 0; q < SM::CP; ++q) {
-            sm.cx[q * SM::SC + q + c] = p.x;
-            sm.cy[q * SM::SC + q + c] = p.y;
-            sm.cd[q * SM::SC + q + c] = p.z;
-        }
-    }
-    // this lane's copy of the columns (see sweep_smem)
-    const int coff = SM::CP == 2 ? ((((threadIdx.x >> 4) ^ (threadIdx.x >> 5)) & 1) * (SM::SC + 1)) : 0;
-    const float *__restrict__ ccx = sm.cx + coff;
-    const float *__restrict__ ccy = sm.cy + coff;
-    const float *__restrict__ ccd = sm.cd + coff;
-    auto col = [&](int c) { return make_float4(ccx[c], ccy[c], ccd[c], 0.0f); };
-    for (int r = threadIdx.x; r <= T; r += NT) {
-        const float4 p = P[min(a0 + r, n - 1)];
-        sm.rx[r] = p.x;
-        sm.ry[r] = p.y;
-        sm.rd[r] = p.z;
-    }
-    __syncthreads();
-
+    const l2o_pt *__restrict__ P = in->pts[in->parity];
+    const int a0 = (int)it.a0;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int cb = w * 64 * K + lane * K;  // this lane's first diagonal, relative to g0
-    const int gmin = g0 + w * 64 * K, gmax = gmin + 64 * K - 1;
-    // candidate (a, a+g) exists iff a + g <= n-2, i.e. step t <= n-2-g-a0
-    const int t_full = max(0, min(T, n - 1 - gmax - a0));  // every lane valid below
-    const int t_end = max(0, min(T, n - 1 - gmin - a0));   // no lane valid from here
-    float *__restrict__ su = sm.u[w];
+    const int bw = (int)it.b0 + w * 64 * K;  // first column of this wave
+    if (bw > n - 2) return;                   // wave-uniform: no candidate column
+    const int bl = bw + lane * K;
+    // rows a <= n-4 have candidates
+    int t_end = min(T, n - 3 - a0);
+    // a wave whose first column reaches past the last row + 2 never meets the
+    // diagonal b = a+1; otherwise rows past its last column - 2 are empty
+    const bool diag = bw < a0 + t_end + 1;
+    if (diag) t_end = min(t_end, min(bw + 64 * K - 1, n - 2) - 1 - a0);
+    const float slack = in->slack;
 
-    float thr = slack - 1.0f;  // (best_gain - 1) + slack with best_gain = 0: only gain <= -1 matters
-    for (int t = lane; t < T; t += 64) su[t] = thr + sm.rd[t + 1];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-    // Two filter tiers.  Tier 1 (every candidate) bounds the gain from below
-    // with Chebyshev distances: nint(d) >= d - 1/2 >= cheb - 1/2, so an exact
-    // gain <= best-1 implies cheb(a,b) + cheb(a+1,b+1) - D[a+1] - D[b+1] <=
-    // best; that sum is an exact FP32 integer and the test against
-    // thr + D[a+1] (thr = best-1+slack) keeps slack-1 of margin for the FP32
-    // rounding of the threshold.  Tier 2 (lanes flagged by tier 1) is the FP32
-    // distance estimate, |exact - estimate| <= slack.  Both tiers only drop
-    // candidates that cannot reach best-1, so the exact pass sees every move
-    // that can win.  A wave ballot of tier 1 is rarely nonzero: the Chebyshev
-    // sum exceeds the two tour edges unless both pairs are spatial neighbours.
-    float qp[K];  // tier-1 bound of the previous step, per diagonal
-    {
-        const float x0 = sm.rx[0], y0 = sm.ry[0];
+    // lane columns and offsets
+    uint32_t cx[K + 1], cy[K + 1], o[K + 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float4 C = col(cb + k);
-            qp[k] = cheb_dist(x0, y0, C.x, C.y);
-        }
+    for (int j = 0; j <= K; ++j) {
+        const l2o_pt c = P[min(bl + j, n - 1)];
+        cx[j] = (uint32_t)c.x;
+        cy[j] = (uint32_t)c.y;
+        // only o_k + o_{k+1} matters; o_0 = W of the first column keeps the
+        // first pair of v_sad_u32 an accumulate (a constant 0 would be
+        // lowered to max - min)
+        if (j == 0) o[0] = (uint32_t)c.w;
+        else o[j] = 0u - (uint32_t)c.w - o[j - 1] + (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
+    }
+    int lim[K];  // diagonal waves: candidate (a0+t, bl+k) exists iff t <= lim[k]
+#pragma unroll
+    for (int k = 0; k < K; ++k) lim[k] = bl + k - 2 - a0;
+
+    // rows: row a0+t at prow[4t .. 4t+3] = {x, y, D, W}
+    const __attribute__((address_space(4))) int *__restrict__ prow =
+        (const __attribute__((address_space(4))) int *)(P + a0);
+
+    uint32_t V[K];  // V(a, k) of the current row a
+    {
+        const uint32_t x0 = (uint32_t)prow[0], y0 = (uint32_t)prow[1];
+#pragma unroll
+        for (int k = 0; k < K; ++k) V[k] = sad(y0, cy[k], sad(x0, cx[k], o[k]));
+    }
+    // cap = the largest gain still worth an exact evaluation.  It starts at
+    // -1 (improving moves only) or at the instance's best key so far, which
+    // other waves of this sweep publish as they go; that key may come from a
+    // larger i, so it is taken inclusively (a tie at a smaller i wins).  The
+    // wave's own best of earlier rows (smaller i) is taken exclusively.
+    auto key_now = [&]() __attribute__((always_inline)) -> unsigned long long {
+        return uniform_u64(__hip_atomic_load(&in->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    };
+    int cap = -1;
+    int tb = 0;                // tier 1: ceil(sqrt2 * (cap + 1))
+    float thr = slack - 1.0f;  // tier 2: cap + slack
+    auto set_cap = [&](int c) __attribute__((always_inline)) {
+        cap = c;
+        tb = __builtin_amdgcn_readfirstlane(tier1_base(c + 1));
+        thr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)c + slack)));
+    };
+    {
+        const unsigned long long g = key_now();
+        if (g != L2O_SENTINEL && key_gain(g) < cap) set_cap(key_gain(g));
     }
     unsigned long long lbest = L2O_SENTINEL;
+    unsigned long long published = L2O_SENTINEL;
     unsigned fb = 0;
 
-    // tier 2 and exact re-evaluation of tier-1 hits (steps t0 .. t0+ns-1);
-    // then the wave's threshold and its per-step thresholds in LDS are
-    // refreshed.  Hit masks are wave ballots (SGPR pairs): lane bits are only
-    // extracted here.
-    auto slow = [&](int t0, int ns, const unsigned long long *hit) {
+    // Steps t0 .. t0+ns-1 hold a tier-1 hit somewhere in the wave: every lane
+    // re-runs tier 1 on its candidates of those steps (same integer values as
+    // the main loop), then tiers 2 and 3 on the survivors; then the wave's
+    // thresholds are refreshed.
+    auto slow = [&](int t0, int ns) __attribute__((always_inline)) {
         for (int s = 0; s < ns; ++s) {
             const int t = t0 + s;
-            const float xa = sm.rx[t], ya = sm.ry[t], xb = sm.rx[t + 1], yb = sm.ry[t + 1], db = sm.rd[t + 1];
-            const float u = su[t];
+            const int xa = prow[4 * t], ya = prow[4 * t + 1];
+            const int xb = prow[4 * t + 4], yb = prow[4 * t + 5], db = prow[4 * t + 6], wr = prow[4 * t + 7];
+            const int u = tb + wr;
+            const float u2 = thr + (float)db;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                if ((hit[s * K + k] >> lane) & 1ull) {
-                    const float4 Ca = col(t + cb + k), Cb = col(t + 1 + cb + k);
-                    const float w2 = (est_dist(xa, ya, Ca.x, Ca.y) + est_dist(xb, yb, Cb.x, Cb.y)) - Cb.z;
-                    if (w2 <= u) {
+                const uint32_t va = sad((uint32_t)ya, cy[k], sad((uint32_t)xa, cx[k], o[k]));
+                const uint32_t vb = sad((uint32_t)yb, cy[k + 1], sad((uint32_t)xb, cx[k + 1], o[k + 1]));
+                if ((int)(va + vb) <= u && (!diag || t <= lim[k])) {
+                    const int dc = P[bl + k + 1].d;
+                    const float w2 = (est_dist(xa, ya, (int)cx[k], (int)cy[k]) +
+                                      est_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1])) - (float)dc;
+                    if (w2 <= u2) {
                         ++fb;
-                        const int gain = exact_dist(xa, ya, Ca.x, Ca.y) + exact_dist(xb, yb, Cb.x, Cb.y) -
-                                         (int)db - (int)Cb.z;
+                        const int gain = exact_dist(xa, ya, (int)cx[k], (int)cy[k]) +
+                                         exact_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1]) - db - dc;
                         if (gain < 0) {
                             const uint32_t i = (uint32_t)(a0 + t + 1);
-                            const uint32_t j = (uint32_t)(a0 + t + 1 + g0 + cb + k);
+                            const uint32_t j = (uint32_t)(bl + k + 1);
                             const unsigned long long key = make_key(gain, i, j);
                             lbest = key < lbest ? key : lbest;
                         }
@@ -233,112 +247,93 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
                 }
             }
         }
-        const unsigned long long wb = wave_min_u64(lbest);
-        if (wb != L2O_SENTINEL) {
-            const float nthr = (float)(key_gain(wb) - 1) + slack;
-            if (nthr != thr) {
-                thr = nthr;
-                __builtin_amdgcn_wave_barrier();
-                for (int t = lane; t < T; t += 64) su[t] = thr + sm.rd[t + 1];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the wave minimum is uniform: keep it, and the thresholds, in SGPRs;
+        // publish it at once so that other waves can tighten their cap
+        const unsigned long long wb = uniform_u64(wave_min_u64(lbest));
+        int c = cap;
+        if (wb < published) {
+            published = wb;
+            c = min(c, key_gain(wb) - 1);
+            if (lane == 0) atomicMin(&in->key, wb);
+        }
+        const unsigned long long g = key_now();
+        if (g != L2O_SENTINEL) c = min(c, key_gain(g));
+        if (c != cap) set_cap(c);
+    };
+
+    // NS steps from t with rows a0+t+1 .. a0+t+NS given in r (x, y, W per
+    // row, wave-uniform): V advances by NS rows; returns the wave's tier-1 hits
+    auto block = [&](int t, const int (*r)[3], auto ns_tag, auto mask_tag)
+                     __attribute__((always_inline)) -> unsigned long long {
+        constexpr int NS = decltype(ns_tag)::value;
+        constexpr bool MASK = decltype(mask_tag)::value;
+        unsigned long long any = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const uint32_t rx = (uint32_t)r[s][0], ry = (uint32_t)r[s][1];
+            const int u = tb + r[s][2];
+            uint32_t Vn[K + 1];
+#pragma unroll
+            for (int j = 0; j <= K; ++j) {
+                Vn[j] = sad(ry, cy[j], sad(rx, cx[j], o[j]));
+                // keeps each V a pair of v_sad_u32 (the last one would
+                // otherwise be reassociated into max/min/sub/add3)
+                asm volatile("" : "+v"(Vn[j]));
             }
+            int m = 0x7FFFFFFF;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                int S = (int)(V[k] + Vn[k + 1]);
+                if constexpr (MASK) S = (t + s <= lim[k]) ? S : 0x7FFFFFFF;
+                m = min(m, S);
+            }
+            any |= __ballot(m <= u);
+#pragma unroll
+            for (int k = 0; k < K; ++k) V[k] = Vn[k];
+        }
+        return any;
+    };
+    auto load_rows = [&](int t, int (*r)[3], auto ns_tag) __attribute__((always_inline)) {
+        constexpr int NS = decltype(ns_tag)::value;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            r[s][0] = prow[4 * (t + s + 1)];
+            r[s][1] = prow[4 * (t + s + 1) + 1];
+            r[s][2] = prow[4 * (t + s + 1) + 3];
         }
     };
 
-    // Main loop.  Row points are wave-uniform: they come from global memory
-    // through the scalar cache (below t_full every row index a0+t+1+s is
-    // < n).  Column words are read through volatile pointers so that each is
-    // one ds_read_b32 off a single per-lane base with an immediate offset:
-    // merged ds_read2 pairs would need a fresh base VGPR per array (their
-    // 8-bit offsets cannot span the column arrays), one VALU add each.
-    const __attribute__((address_space(4))) float *__restrict__ prow =
-        (const __attribute__((address_space(4))) float *)(P + a0 + 1);  // row t at prow[4t]
-    typedef const volatile __attribute__((address_space(3))) float lds_vf;
-    lds_vf *vcx = (lds_vf *)(ccx + cb + 1), *vcy = (lds_vf *)(ccy + cb + 1), *vcd = (lds_vf *)(ccd + cb + 1);
-    int t = 0;
-    const int t_main = t_full - t_full % UF;
-    for (; t < t_main; t += UF) {
-        float x[UF], y[UF], U[UF];
+    // the rows of the next iteration are requested before this one computes
+    // (the last iteration re-reads its own rows: every address stays < n)
+    auto run = [&](auto mask_tag) __attribute__((always_inline)) {
+        using UFc = std::integral_constant<int, UF>;
+        int t = 0;
+        const int t_main = t_end - t_end % UF;
+        if (t_main > 0) {
+            int cur[UF][3];
+            load_rows(0, cur, UFc{});
+            for (; t < t_main; t += UF) {
+                int nxt[UF][3];
+                load_rows(min(t + UF, t_main - UF), nxt, UFc{});
+                if (block(t, cur, UFc{}, mask_tag)) slow(t, UF);
 #pragma unroll
-        for (int s = 0; s < UF; ++s) {
-            x[s] = prow[4 * (t + s)];
-            y[s] = prow[4 * (t + s) + 1];
-            U[s] = su[t + s];
-        }
-        float4 C[UF + K - 1];
+                for (int s = 0; s < UF; ++s)
 #pragma unroll
-        for (int j = 0; j < UF + K - 1; ++j) C[j] = make_float4(vcx[t + j], vcy[t + j], vcd[t + j], 0.0f);
-        unsigned long long hit[UF * K];
-        unsigned long long any = 0;
-#pragma unroll
-        for (int s = 0; s < UF; ++s) {
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float q = cheb_dist(x[s], y[s], C[s + k].x, C[s + k].y);
-                const float w2 = (qp[k] + q) - C[s + k].z;
-                qp[k] = q;
-                hit[s * K + k] = __ballot(w2 <= U[s]);
-                any |= hit[s * K + k];
+                    for (int q = 0; q < 3; ++q) cur[s][q] = nxt[s][q];
             }
         }
-        if (any) slow(t, UF, hit);
-    }
-    int lim[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) lim[k] = n - 2 - a0 - (g0 + cb + k);
-    for (; t < t_end; ++t) {
-        const float x = sm.rx[t + 1], y = sm.ry[t + 1], U = su[t];
-        unsigned long long hit[K];
-        unsigned long long any = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float4 C = col(t + 1 + cb + k);
-            const float q = cheb_dist(x, y, C.x, C.y);
-            const float w2 = (qp[k] + q) - C.z;
-            qp[k] = q;
-            hit[k] = __ballot((w2 <= U) && (t <= lim[k]));
-            any |= hit[k];
+        for (; t < t_end; ++t) {
+            int r1[1][3];
+            load_rows(t, r1, std::integral_constant<int, 1>{});
+            if (block(t, r1, std::integral_constant<int, 1>{}, mask_tag)) slow(t, 1);
         }
-        if (any) slow(t, 1, hit);
-    }
+    };
+    if (diag) run(std::true_type{});
+    else run(std::false_type{});
 
-    const unsigned long long wb = wave_min_u64(lbest);
+    // every improvement was published by the slow path
     const unsigned wfb = wave_sum_u32(fb);
-    if (lane == 0) {
-        sm.key[w] = wb;
-        sm.fb[w] = wfb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long key = sm.key[0];
-        unsigned f = sm.fb[0];
-#pragma unroll
-        for (int k = 1; k < WAVES; ++k) {
-            key = sm.key[k] < key ? sm.key[k] : key;
-            f += sm.fb[k];
-        }
-        if (key != L2O_SENTINEL) {
-            const unsigned long long cur = __hip_atomic_load(&in->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (key < cur) atomicMin(&in->key, key);
-        }
-        if (f) atomicAdd(&in->fallbacks, (unsigned long long)f);
-    }
-}
-
-// One workgroup per work item.  (A persistent variant that loops over items
-// fetched from an atomic ticket is NOT used: with a lane-0 ticket fetch and
-// lane-0 key atomics inside the item loop, the AMDGPU control-flow
-// structurizer emits an exec-masked loop nest in which lanes 1..63 re-enter
-// the item body on a stale ticket while lane 0 waits outside -- a livelock.
-// Dispatch-round quantization is handled instead by short items, t = 64.)
-template <int K, int T, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__restrict__ items,
-                                                      l2o_inst *__restrict__ insts)
-{
-    __shared__ sweep_smem<K, T, WAVES> sm;
-    sweep_item<K, T, WAVES>(items[blockIdx.x], insts, sm);
+    if (lane == 0 && wfb) atomicAdd(&in->fallbacks, (unsigned long long)wfb);
 }
 
 // ---------------------------------------------------------------------------
@@ -375,36 +370,47 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
     const uint32_t J = (uint32_t)key & 0xFFFFFu;
     const uint32_t L = J - i;
     const uint32_t par = in->parity;
-    const float4 *__restrict__ P = in->pts[par];
+    const l2o_pt *__restrict__ P = in->pts[par];
     const uint32_t *__restrict__ R = in->tour[par];
-    float4 *__restrict__ Pn = in->pts[par ^ 1];
+    l2o_pt *__restrict__ Pn = in->pts[par ^ 1];
     uint32_t *__restrict__ Rn = in->tour[par ^ 1];
     const uint32_t k0 = it.chunk * L2O_APPLY_CHUNK;
     const uint32_t k1 = min(n, k0 + L2O_APPLY_CHUNK);
     for (uint32_t k = k0 + threadIdx.x; k < k1; k += L2O_APPLY_THREADS) {
-        uint32_t src;
-        float d;
+        uint32_t src, de;  // source point; position whose edge (length d, bound w) lands here
+        bool fresh = false;
         if (k < L) {
             src = i + k;
-            if (k == 0) {
-                const float4 a = P[i], b = P[J];
-                d = (float)exact_dist(a.x, a.y, b.x, b.y);
-            } else {
-                d = P[src].z;
-            }
+            de = src;
+            fresh = k == 0;  // new edge P[i] -> P[J]
         } else if (k == L) {
             src = i - 1;
-            const float4 a = P[J - 1], b = P[i - 1];
-            d = (float)exact_dist(a.x, a.y, b.x, b.y);
+            de = src;
+            fresh = true;    // new edge P[J-1] -> P[i-1]
         } else if (k < L + i) {
             src = i - 1 - (k - L);
-            d = P[src + 1].z;
+            de = src + 1;
         } else {
             src = n - 1 + L + i - k;
-            d = P[src + 1 == n ? 0 : src + 1].z;
+            de = src + 1 == n ? 0 : src + 1;
         }
-        const float4 p = P[src];
-        Pn[k] = make_float4(p.x, p.y, d, 0.0f);
+        const l2o_pt p = P[src];
+        int d, w;
+        if (fresh) {
+            const l2o_pt q = k == 0 ? P[J] : P[J - 1];
+            d = exact_dist(p.x, p.y, q.x, q.y);
+            w = ceil_sqrt2(d);
+        } else {
+            const l2o_pt e = P[de];
+            d = e.d;
+            w = e.w;
+        }
+        l2o_pt r;
+        r.x = p.x;
+        r.y = p.y;
+        r.d = d;
+        r.w = w;
+        Pn[k] = r;
         Rn[k] = R[src];
     }
     __threadfence();
@@ -502,9 +508,9 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
         }
     }
     __syncthreads();
-    const float4 *__restrict__ P = in->pts[par];
+    const l2o_pt *__restrict__ P = in->pts[par];
     const uint32_t *__restrict__ R = in->tour[par];
-    float4 *__restrict__ BP = in->bpts;
+    l2o_pt *__restrict__ BP = in->bpts;
     uint32_t *__restrict__ BR = in->btour;
     if (s_accept) {
         for (uint32_t k = threadIdx.x; k < n; k += L2O_ILS_THREADS) {
@@ -533,16 +539,21 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
         if (k < c) return a + (k - a - (c - b));
         return k;
     };
-    float4 *__restrict__ Pn = in->pts[0];
+    l2o_pt *__restrict__ Pn = in->pts[0];
     uint32_t *__restrict__ Rn = in->tour[0];
     unsigned long long sum = 0;
     for (uint32_t k = threadIdx.x; k < n; k += L2O_ILS_THREADS) {
         const uint32_t s = src_of(k);
         const uint32_t sp = src_of(k == 0 ? n - 1 : k - 1);
-        const float4 p = BP[s], q = BP[sp];
+        const l2o_pt p = BP[s], q = BP[sp];
         const int d = exact_dist(q.x, q.y, p.x, p.y);
         sum += (unsigned long long)d;
-        Pn[k] = make_float4(p.x, p.y, (float)d, 0.0f);
+        l2o_pt r;
+        r.x = p.x;
+        r.y = p.y;
+        r.d = d;
+        r.w = ceil_sqrt2(d);
+        Pn[k] = r;
         Rn[k] = BR[s];
     }
 #pragma unroll
@@ -573,11 +584,11 @@ __global__ __launch_bounds__(256) void k_length(const l2o_inst *__restrict__ ins
                                                 unsigned long long *__restrict__ out)
 {
     const l2o_inst *in = insts + slot;
-    const float4 *P = in->pts[in->parity];
+    const l2o_pt *P = in->pts[in->parity];
     const uint32_t n = in->n;
     unsigned long long sum = 0;
     for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        const float4 p = P[k], q = P[k == 0 ? n - 1 : k - 1];
+        const l2o_pt p = P[k], q = P[k == 0 ? n - 1 : k - 1];
         sum += (unsigned long long)exact_dist(q.x, q.y, p.x, p.y);
     }
 #pragma unroll
@@ -625,9 +636,9 @@ namespace l2o {
 // tile shapes compiled into the library (k diagonals per lane, t steps,
 // waves per workgroup); the autotuner chooses among these per n bucket
 #define L2O_SWEEP_SHAPES(X)                                                                          \
-    X(1, 64, 1) X(1, 128, 1) X(2, 64, 1) X(2, 128, 1) X(4, 64, 1) X(4, 128, 1) X(1, 128, 2)          \
-    X(2, 64, 2) X(2, 128, 2) X(4, 64, 2) X(4, 128, 2) X(1, 64, 4) X(1, 128, 4) X(1, 256, 4)            \
-    X(2, 64, 4) X(2, 128, 4) X(2, 256, 4) X(4, 64, 4) X(4, 128, 4) X(4, 256, 4)
+    X(1, 32, 1) X(1, 64, 1) X(2, 64, 1) X(2, 128, 1) X(4, 64, 1) X(4, 128, 1) X(4, 256, 1)           \
+    X(6, 128, 1) X(6, 256, 1) X(8, 64, 1) X(8, 128, 1) X(8, 256, 1) X(4, 128, 2) X(8, 128, 2)         \
+    X(8, 256, 2) X(1, 64, 4) X(2, 64, 4) X(4, 128, 4) X(8, 128, 4) X(8, 256, 4)
 
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \

@@ -6,13 +6,26 @@
 
 namespace l2o {
 
-// Sweep tile shape: each workgroup has `waves` wave64s, each lane walks `k`
-// adjacent diagonals for `t` steps.  A workgroup covers 64*waves*k diagonals.
+// ceil(sqrt(2) * d) for 0 <= d < 2^24.  For d >= 1, sqrt(2) d is irrational,
+// so the ceiling is isqrt(2 d^2) + 1; isqrt is exact after the correction.
+__host__ __device__ inline int32_t ceil_sqrt2(int32_t d)
+{
+    if (d <= 0) return 0;
+    const long long s = 2ll * d * d;
+    long long k = (long long)sqrt((double)s);
+    if (k * k > s) k -= 1;
+    else if ((k + 1) * (k + 1) <= s) k += 1;
+    return (int32_t)(k + 1);
+}
+
+// Sweep tile shape: each workgroup has `waves` wave64s, each lane owns `k`
+// adjacent candidate columns for `t` rows.  A workgroup covers 64*waves*k
+// columns.
 struct sweep_cfg {
     int k;
     int t;
     int waves;
-    int diag() const { return 64 * waves * k; }
+    int cols() const { return 64 * waves * k; }
 };
 
 bool sweep_cfg_supported(const sweep_cfg &c);

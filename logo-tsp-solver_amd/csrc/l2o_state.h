@@ -2,9 +2,10 @@
 // the gfx950 kernels.
 //
 // Per instance (n cities) HBM holds, 16-byte aligned:
-//   pts[2][n]  float4 {x, y, D, 0}   tour-ordered coordinates (ping-pong);
-//                                     D[k] = exact d(P[k-1], P[k]), D[0] the
-//                                     closing edge d(P[n-1], P[0])
+//   pts[2][n]  l2o_pt {x, y, D, W}   tour-ordered points (ping-pong): integer
+//                                     coordinates shifted into [0, 2^22),
+//                                     D[k] = exact nint(d(P[k-1], P[k])), D[0]
+//                                     the closing edge, W[k] = ceil(sqrt(2) D[k])
 //   tour[2][n] uint32                 city ids in the same order (ping-pong)
 //   bpts[n], btour[n]                 best tour of an ILS run
 //   log[log_cap]                      applied moves {i, J, gain}
@@ -29,6 +30,12 @@ enum l2o_phase : uint32_t {
     L2O_DONE = 3,     // finished (result ready)
 };
 
+struct __attribute__((aligned(16))) l2o_pt {
+    int32_t x, y;  // coordinates, 0 <= x, y < 2^22
+    int32_t d;     // exact length of the edge into this point
+    int32_t w;     // ceil(sqrt(2) * d), the edge's tier-1 sweep bound
+};
+
 struct l2o_move {
     uint32_t i;
     uint32_t j;
@@ -37,9 +44,9 @@ struct l2o_move {
 
 struct l2o_inst {
     // ---- fixed at admission
-    float4 *pts[2];
+    l2o_pt *pts[2];
     uint32_t *tour[2];
-    float4 *bpts;
+    l2o_pt *bpts;
     uint32_t *btour;
     l2o_move *log;
     unsigned long long *rlog;  // ILS: best length after the first LO and each round
@@ -63,12 +70,12 @@ struct l2o_inst {
     unsigned long long rng[4];
 };
 
-// One workgroup of the sweep kernel: diagonals g in [g0, g0 + DIAG) of the
-// (a, g = b - a) triangle, steps a in [a0, a0 + T).
+// One workgroup of the sweep kernel: rows a in [a0, a0 + T) and columns b in
+// [b0, b0 + 64 * waves * k) of the candidate triangle a + 2 <= b <= n - 2.
 struct l2o_sweep_item {
     uint32_t inst;
-    uint32_t g0;
     uint32_t a0;
+    uint32_t b0;
     uint32_t pad;
 };
 

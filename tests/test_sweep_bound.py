@@ -1,0 +1,135 @@
+"""Pins the sweep kernel's first filter tier (logo-tsp-solver_amd/csrc/l2o_kernels.hip)
+on the CPU, in exact integer arithmetic independent of the kernel.
+
+The kernel skips a candidate (a, b) unless
+    L1(a,b) + L1(a+1,b+1) <= ceil(sqrt2 * best) + W[a+1] + W[b+1],
+W[k] = ceil(sqrt(2) * D[k]), best = the wave's best exact gain so far (<= 0).
+That is sound only if every candidate that could still win (exact gain
+G <= best - 1) passes.  The claim rests on nint(d) >= d - 1/2 and
+|dx| + |dy| <= sqrt(2) d; these tests check the claim itself, over every
+candidate of small instances, at coordinate spans up to the 2^22 limit and on
+tours with many near-ties, and check the integer ceil(sqrt(2) d) the kernel and
+host compute.  A mistake such as a missing 1/2 per rounding, a floor for a
+ceil, or the row and column terms of the wrong edges fails here."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def ceil_sqrt2_ref(d):
+    """Smallest m with m^2 >= 2 d^2, by definition."""
+    m = math.isqrt(2 * d * d)
+    return m if m * m >= 2 * d * d else m + 1
+
+
+def ceil_sqrt2_kernel(d):
+    """The kernel's and host's formula (l2o_launch.h): 0 for d = 0, else
+    isqrt(2 d^2) + 1 -- valid because 2 d^2 is never a perfect square."""
+    return 0 if d <= 0 else math.isqrt(2 * d * d) + 1
+
+
+def tier1_base(best):
+    """ceil(sqrt(2) * best) for best <= 0, as the kernel computes it."""
+    return 0 if best == 0 else 1 - ceil_sqrt2_kernel(-best)
+
+
+def test_ceil_sqrt2_matches_definition():
+    rng = np.random.default_rng(3)
+    ds = list(range(0, 5000)) + [int(v) for v in rng.integers(1, 1 << 24, 20000)] + [(1 << 24) - 1]
+    for d in ds:
+        assert ceil_sqrt2_kernel(d) == ceil_sqrt2_ref(d), d
+        assert ceil_sqrt2_kernel(d) >= math.sqrt(2) * d
+
+
+def test_tier1_base_is_ceiling():
+    for best in range(0, -20000, -1):
+        tb = tier1_base(best)
+        # tb is the smallest integer >= sqrt(2) * best
+        assert tb <= 0 and tb * tb <= 2 * best * best
+        assert (tb - 1) < math.sqrt(2) * best <= tb
+
+
+def _gains_and_l1(xy, r):
+    """Exact gain G and L1 sum of every candidate (a, b), a+2 <= b <= n-2,
+    plus the W terms of its row and column edge (python integers)."""
+    n = len(r)
+    X = [int(xy[2 * c]) for c in r]
+    Y = [int(xy[2 * c + 1]) for c in r]
+
+    def nint(p, q):
+        dx, dy = X[p] - X[q], Y[p] - Y[q]
+        return (math.isqrt(4 * (dx * dx + dy * dy)) + 1) // 2
+
+    def l1(p, q):
+        return abs(X[p] - X[q]) + abs(Y[p] - Y[q])
+
+    D = [nint(k - 1 if k else n - 1, k) for k in range(n)]
+    W = [ceil_sqrt2_kernel(d) for d in D]
+    out = []
+    for a in range(0, n - 3):
+        for b in range(a + 2, n - 1):
+            G = nint(a, b) + nint(a + 1, b + 1) - D[a + 1] - D[b + 1]
+            out.append((G, l1(a, b) + l1(a + 1, b + 1), W[a + 1] + W[b + 1]))
+    return out
+
+
+def _check(xy, r):
+    cands = _gains_and_l1(xy, r)
+    gains = sorted({g for g, _, _ in cands})
+    # thresholds at no move found yet, at the optimum, and in between
+    bests = {0, min(min(gains), 0)}
+    bests.update(g for g in gains[:: max(1, len(gains) // 25)] if g <= 0)
+    for best in bests:
+        tb = tier1_base(best)
+        for G, L, Wsum in cands:
+            if G <= best - 1:
+                assert L <= tb + Wsum, (best, G, L, Wsum)
+    return len(cands)
+
+
+@pytest.mark.parametrize("n,side,kind,seed", [
+    (60, 10**4, "u", 1), (80, 10**6, "c", 2), (70, (1 << 22) - 1, "u", 3), (64, 40, "u", 4), (90, 3, "u", 5),
+])
+def test_tier1_never_drops_a_winner_random_tours(n, side, kind, seed):
+    xy = O.gen_uniform(n, side, seed) if kind == "u" else O.gen_clustered(n, side, seed)
+    for s in range(2):
+        assert _check(xy, O.random_perm(n, 100 + s)) > 0
+
+
+def test_tier1_never_drops_a_winner_near_optimum():
+    # near a local optimum the best gains are small and many candidates sit
+    # close to the threshold
+    for n, side in [(120, 10**4), (100, (1 << 22) - 1), (110, 25)]:
+        xy = O.gen_uniform(n, side, n)
+        r = O.nn_tour(xy, 0)
+        _check(xy, r)
+        r2 = O.local_search(xy, r, max_moves=max(1, n // 10))[0]
+        _check(xy, r2)
+
+
+def test_tier1_tight_instance():
+    # A=(40,40) B=(0,2) C=(41,41) E=(39,41), tour A B C E: the one candidate
+    # has both new edges at 45 degrees (L1 = sqrt2 d exactly), n(B,E) =
+    # nint(39 sqrt2 = 55.15) rounds down, and G = 1 + 55 - 55 - 2 = -1.  Since
+    # nint rounds half up, d < nint(d) + 1/2 strictly and the bound always
+    # keeps a margin of at least 1; here the margin is exactly 1, so any
+    # threshold lower by 2 or more (a dropped rounding term, a W floored) fails.
+    xy = O.xy_of(np.array([40, 0, 41, 39], np.int32), np.array([40, 2, 41, 41], np.int32))
+    r = np.arange(4, dtype=np.uint32)
+    (G, L, Wsum), = _gains_and_l1(xy, r)
+    assert G == -1 and L == 80 and Wsum == 81
+    assert tier1_base(0) + Wsum - L == 1
+    _check(xy, r)
+
+
+def test_tier1_extreme_geometry():
+    # collinear points, duplicates and the largest allowed span
+    span = (1 << 22) - 1
+    x = np.array([0, span, 0, span, 1, span - 1, 0, 0, span, 7, 7, 7, span // 2, span // 3, 2, span], np.int32)
+    y = np.array([0, span, span, 0, 1, span - 1, 0, 5, 3, 7, 7, 8, span // 2, 2 * span // 3, span, 1], np.int32)
+    xy = O.xy_of(x, y)
+    for s in range(6):
+        _check(xy, O.random_perm(len(x), s))
