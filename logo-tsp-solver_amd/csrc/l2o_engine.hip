@@ -389,6 +389,22 @@ void drop_graph(logo2opt_engine *e)
     e->gsteps = 0;
 }
 
+// candidates (a, b), a+2 <= b <= n-2, of a sweep item (closed form)
+uint64_t item_work(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
+{
+    const int64_t b0 = it.b0, a0 = it.a0;
+    const int64_t a_end = std::min<int64_t>(a0 + c.t, (int64_t)n - 3);  // rows a0 .. a_end-1
+    const int64_t bhi = std::min<int64_t>(b0 + c.cols() - 1, (int64_t)n - 2);
+    if (a_end <= a0 || bhi < b0) return 0;
+    // rows a <= b0-2 see all of [b0, bhi]
+    const int64_t r1 = std::max<int64_t>(0, std::min(a_end, b0 - 1) - a0);
+    uint64_t w = (uint64_t)r1 * (uint64_t)(bhi - b0 + 1);
+    // rows a >= b0-1 see [a+2, bhi]: bhi-1-a candidates while a <= bhi-2
+    const int64_t lo = std::max(a0, b0 - 1), hi = std::min(a_end, bhi - 1);  // a in [lo, hi)
+    if (hi > lo) w += (uint64_t)((2 * (bhi - 1) - lo - (hi - 1)) * (hi - lo) / 2);
+    return w;
+}
+
 int rebuild_lists(logo2opt_engine *e)
 {
     std::map<std::tuple<int, int, int>, std::vector<l2o_sweep_item>> by_cfg;
@@ -426,7 +442,19 @@ int rebuild_lists(logo2opt_engine *e)
         // Items are dispatched in list order (largest instances first), so at
         // any moment the device works on a window of consecutive items of one
         // or a few instances, whose points stay hot in every XCD's L2.
-        const auto &v = kv.second;
+        // longest items first (candidates per item: full tiles, then the
+        // partial tiles at the diagonal and the right and bottom edges), so
+        // the dispatch tail is made of the short ones; stable, so equal items
+        // keep their instance order
+        auto &v = kv.second;
+        std::vector<uint64_t> work(v.size());
+        for (size_t q = 0; q < v.size(); ++q) work[q] = item_work(v[q], c, e->slots[v[q].inst].n);
+        std::vector<uint32_t> idx(v.size());
+        for (uint32_t q = 0; q < idx.size(); ++q) idx[q] = q;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
+        std::vector<l2o_sweep_item> sorted(v.size());
+        for (size_t q = 0; q < idx.size(); ++q) sorted[q] = v[idx[q]];
+        v.swap(sorted);
         const uint32_t cnt = (uint32_t)v.size();
         if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;
         e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size()});

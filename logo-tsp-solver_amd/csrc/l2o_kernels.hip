@@ -115,6 +115,29 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v)
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// point records in global memory (address space 1: the pointers come from
+// the instance record, so the space is stated for global_* instructions)
+typedef __attribute__((address_space(1))) l2o_pt gpt;
+typedef __attribute__((address_space(1))) uint32_t gu32;
+
+__device__ __forceinline__ l2o_pt ld_pt(const gpt *P, uint32_t k)
+{
+    l2o_pt r;
+    r.x = P[k].x;
+    r.y = P[k].y;
+    r.d = P[k].d;
+    r.w = P[k].w;
+    return r;
+}
+
+__device__ __forceinline__ void st_pt(gpt *P, uint32_t k, const l2o_pt r)
+{
+    P[k].x = r.x;
+    P[k].y = r.y;
+    P[k].d = r.d;
+    P[k].w = r.w;
+}
+
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 {
 #pragma unroll
@@ -146,9 +169,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
     static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
     const l2o_sweep_item it = items[blockIdx.x];
     l2o_inst *in = insts + it.inst;
-    if (in->phase != L2O_LO) return;
-    const int n = (int)in->n;
-    const l2o_pt *__restrict__ P = in->pts[in->parity];
+    // The fields this kernel reads but no wave writes come in one batch of
+    // scalar loads before any branch (constant address space: other kernels
+    // and the host write them, and the scalar cache starts clean), so the
+    // prologue is four dependent loads deep: item, instance, data.
+    const __attribute__((address_space(4))) l2o_inst *ci = (const __attribute__((address_space(4))) l2o_inst *)in;
+    const uint32_t phase = ci->phase, par = ci->parity;
+    const int n = (int)ci->n;
+    const float slack = ci->slack;
+    l2o_pt *const p0 = ci->pts[0], *const p1 = ci->pts[1];
+    // (an empty asm that consumes them keeps the loads above the branches)
+    asm volatile("" ::"s"(phase), "s"(par), "s"(n), "s"(slack), "s"(p0), "s"(p1), "s"(it.a0), "s"(it.b0));
+    if (phase != L2O_LO) return;
+    const gpt *__restrict__ P = (const gpt *)(par ? p1 : p0);
     const int a0 = (int)it.a0;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -161,13 +194,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
     // diagonal b = a+1; otherwise rows past its last column - 2 are empty
     const bool diag = bw < a0 + t_end + 1;
     if (diag) t_end = min(t_end, min(bw + 64 * K - 1, n - 2) - 1 - a0);
-    const float slack = in->slack;
 
     // lane columns and offsets
     uint32_t cx[K + 1], cy[K + 1], o[K + 1];
 #pragma unroll
     for (int j = 0; j <= K; ++j) {
-        const l2o_pt c = P[min(bl + j, n - 1)];
+        const l2o_pt c = ld_pt(P, (uint32_t)min(bl + j, n - 1));
         cx[j] = (uint32_t)c.x;
         cy[j] = (uint32_t)c.y;
         // only o_k + o_{k+1} matters; o_0 = W of the first column keeps the
@@ -176,9 +208,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
         if (j == 0) o[0] = (uint32_t)c.w;
         else o[j] = 0u - (uint32_t)c.w - o[j - 1] + (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
     }
-    int lim[K];  // diagonal waves: candidate (a0+t, bl+k) exists iff t <= lim[k]
-#pragma unroll
-    for (int k = 0; k < K; ++k) lim[k] = bl + k - 2 - a0;
+    // diagonal waves: candidate (a0+t, bl+k) exists iff t - lim0 <= k
+    const int lim0 = bl - 2 - a0;
 
     // rows: row a0+t at prow[4t .. 4t+3] = {x, y, D, W}
     const __attribute__((address_space(4))) int *__restrict__ prow =
@@ -229,7 +260,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
             for (int k = 0; k < K; ++k) {
                 const uint32_t va = sad((uint32_t)ya, cy[k], sad((uint32_t)xa, cx[k], o[k]));
                 const uint32_t vb = sad((uint32_t)yb, cy[k + 1], sad((uint32_t)xb, cx[k + 1], o[k + 1]));
-                if ((int)(va + vb) <= u && (!diag || t <= lim[k])) {
+                if ((int)(va + vb) <= u && (!diag || t - lim0 <= k)) {
                     const int dc = P[bl + k + 1].d;
                     const float w2 = (est_dist(xa, ya, (int)cx[k], (int)cy[k]) +
                                       est_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1])) - (float)dc;
@@ -284,7 +315,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__re
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 int S = (int)(V[k] + Vn[k + 1]);
-                if constexpr (MASK) S = (t + s <= lim[k]) ? S : 0x7FFFFFFF;
+                if constexpr (MASK) S = (t + s - lim0 <= k) ? S : 0x7FFFFFFF;
                 m = min(m, S);
             }
             any |= __ballot(m <= u);
@@ -370,48 +401,62 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
     const uint32_t J = (uint32_t)key & 0xFFFFFu;
     const uint32_t L = J - i;
     const uint32_t par = in->parity;
-    const l2o_pt *__restrict__ P = in->pts[par];
-    const uint32_t *__restrict__ R = in->tour[par];
-    l2o_pt *__restrict__ Pn = in->pts[par ^ 1];
-    uint32_t *__restrict__ Rn = in->tour[par ^ 1];
+    const gpt *__restrict__ P = (const gpt *)in->pts[par];
+    const gu32 *__restrict__ R = (const gu32 *)in->tour[par];
+    gpt *__restrict__ Pn = (gpt *)in->pts[par ^ 1];
+    gu32 *__restrict__ Rn = (gu32 *)in->tour[par ^ 1];
     const uint32_t k0 = it.chunk * L2O_APPLY_CHUNK;
     const uint32_t k1 = min(n, k0 + L2O_APPLY_CHUNK);
-    for (uint32_t k = k0 + threadIdx.x; k < k1; k += L2O_APPLY_THREADS) {
-        uint32_t src, de;  // source point; position whose edge (length d, bound w) lands here
-        bool fresh = false;
+    // A thread's CHUNK/THREADS elements are independent: their sources are
+    // computed first and all their loads issued before the first store.
+    constexpr int PER = L2O_APPLY_CHUNK / L2O_APPLY_THREADS;
+    uint32_t src[PER], de[PER];  // source point; position whose edge (d, w) lands here
+    bool ok[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const uint32_t k = k0 + threadIdx.x + e * L2O_APPLY_THREADS;
+        ok[e] = k < k1;
+        uint32_t sr, dr;
         if (k < L) {
-            src = i + k;
-            de = src;
-            fresh = k == 0;  // new edge P[i] -> P[J]
+            sr = i + k;
+            dr = sr;
         } else if (k == L) {
-            src = i - 1;
-            de = src;
-            fresh = true;    // new edge P[J-1] -> P[i-1]
+            sr = i - 1;
+            dr = sr;
         } else if (k < L + i) {
-            src = i - 1 - (k - L);
-            de = src + 1;
+            sr = i - 1 - (k - L);
+            dr = sr + 1;
         } else {
-            src = n - 1 + L + i - k;
-            de = src + 1 == n ? 0 : src + 1;
+            sr = n - 1 + L + i - k;
+            dr = sr + 1 == n ? 0 : sr + 1;
         }
-        const l2o_pt p = P[src];
-        int d, w;
-        if (fresh) {
-            const l2o_pt q = k == 0 ? P[J] : P[J - 1];
-            d = exact_dist(p.x, p.y, q.x, q.y);
-            w = ceil_sqrt2(d);
+        src[e] = ok[e] ? sr : 0;
+        de[e] = ok[e] ? dr : 0;
+    }
+    l2o_pt p[PER], q[PER];
+    uint32_t rid[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        p[e] = ld_pt(P, src[e]);
+        q[e] = ld_pt(P, de[e]);
+        rid[e] = R[src[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const uint32_t k = k0 + threadIdx.x + e * L2O_APPLY_THREADS;
+        if (!ok[e]) continue;
+        l2o_pt r = p[e];
+        if (k == 0 || k == L) {
+            // the two new edges: P[i] -> P[J] at position 0, P[J-1] -> P[i-1] at L
+            const l2o_pt o = ld_pt(P, k == 0 ? J : J - 1);
+            r.d = exact_dist(r.x, r.y, o.x, o.y);
+            r.w = ceil_sqrt2(r.d);
         } else {
-            const l2o_pt e = P[de];
-            d = e.d;
-            w = e.w;
+            r.d = q[e].d;
+            r.w = q[e].w;
         }
-        l2o_pt r;
-        r.x = p.x;
-        r.y = p.y;
-        r.d = d;
-        r.w = w;
-        Pn[k] = r;
-        Rn[k] = R[src];
+        st_pt(Pn, k, r);
+        Rn[k] = rid[e];
     }
     __threadfence();
     __syncthreads();
@@ -637,8 +682,9 @@ namespace l2o {
 // waves per workgroup); the autotuner chooses among these per n bucket
 #define L2O_SWEEP_SHAPES(X)                                                                          \
     X(1, 32, 1) X(1, 64, 1) X(2, 64, 1) X(2, 128, 1) X(4, 64, 1) X(4, 128, 1) X(4, 256, 1)           \
-    X(6, 128, 1) X(6, 256, 1) X(8, 64, 1) X(8, 128, 1) X(8, 256, 1) X(4, 128, 2) X(8, 128, 2)         \
-    X(8, 256, 2) X(1, 64, 4) X(2, 64, 4) X(4, 128, 4) X(8, 128, 4) X(8, 256, 4)
+    X(6, 64, 1) X(6, 128, 1) X(8, 64, 1) X(8, 128, 1) X(8, 256, 1) X(4, 64, 2) X(4, 128, 2)           \
+    X(6, 64, 2) X(8, 64, 2) X(8, 128, 2) X(8, 256, 2) X(2, 64, 4) X(4, 64, 4) X(6, 64, 4)             \
+    X(8, 64, 4) X(8, 128, 4)
 
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \
