@@ -41,6 +41,10 @@ METRIC = "2-opt move evals/sec + instances/sec batched vs plain, bit-exact tour 
 # 2 new edges x (2 sub + mul + fma(2) + sqrt + add); the kernel's own filter
 # does less (integer L1 bounds, see l2o_kernels.hip) and is priced the same
 FLOP_PER_EVAL = 14
+# HBM bytes per sweep launch of the 33,810-city search, from two rocprofv3 --pmc runs of this
+# benchmark (FETCH_SIZE 2,730.8 KiB x 2.0 + WRITE_SIZE 4.99 KiB x 1.0, factors from a 256 MiB copy;
+# profiles/k_sweep_config2_counters.md).  Reported only for the kernel that was profiled.
+PROFILED_TRAFFIC = {"l2o::k_sweep<8,64,4>": 2730.83 * 1024 * 2.0 + 4.9853 * 1024 * 1.0}
 REF_FLOP_PER_EVAL = 36    # the reference's GPU GFLOPS convention (solver.c:305-306)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 
@@ -280,9 +284,10 @@ def main():
     # convergence inside a sync batch do no work and are counted too)
     flop_per_launch = FLOP_PER_EVAL * pairs(n) * work_sweeps / launches
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
+    kname = sweep_kernel_name(n)
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                "kernel": sweep_kernel_name(n), "avg_kernel_ms": avg_ms, "launches": launches,
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": PROFILED_TRAFFIC.get(kname),
+                "kernel": kname, "avg_kernel_ms": avg_ms, "launches": launches,
                 "evals_per_s_kernel": pairs(n) * work_sweeps / launches / (avg_ms * 1e-3),
                 "ref_convention_tflops": achieved * REF_FLOP_PER_EVAL / FLOP_PER_EVAL}
     ctx.close()

@@ -236,7 +236,7 @@ int logo2opt_server_replay(logo2opt_server *s, uint32_t count, const double *arr
 /* Sweep tile shapes (autotuning)                                            */
 /* ======================================================================== */
 /* A shape is {k, t, waves}: a workgroup of `waves` wave64s whose lanes each
- * walk k adjacent diagonals of the candidate triangle for t steps.  The
+ * own k adjacent candidate columns of the triangle for t rows.  The
  * reference hard-codes <<<64,1024>>> and 2048-point tiles
  * (CUDASolver.cpp:122-128, cuda_defs.h:24-37); here the shape is chosen per
  * n bucket from a table written by the autotuner (tune.py). */

@@ -12,7 +12,7 @@ use from its own directory:
                            an engine holding enough instances of that size to
                            fill the GPU (logo2opt_engine_time_sweep), and the
                            time per candidate move decides
-A shape is {k diagonals per lane, t steps, waves per workgroup}.  Bucket edges
+A shape is {k columns per lane, t rows, waves per workgroup}.  Bucket edges
 are the geometric midpoints between the representative sizes.
 """
 import argparse
@@ -74,7 +74,7 @@ def tune_batch(reps=BATCH_REPS, seed=2, verbose=True):
         best.append((n, b))
         if verbose:
             ranked = sorted(times.items(), key=lambda kv: kv[1])[:4]
-            print(f"batch  n={n:>6} (x{m}): " + "  ".join(f"{s}:{1e9 * ms * 1e-3:.3f}ns/eval" for s, ms in ranked),
+            print(f"batch  n={n:>6} (x{m}): " + "  ".join(f"{s}:{ms * 1e-3 * 1e12:.3f}ps/eval" for s, ms in ranked),
                   flush=True)
     return best
 
