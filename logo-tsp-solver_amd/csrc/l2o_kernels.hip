@@ -161,8 +161,10 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 // candidate is masked by its own step limit.
 #define L2O_T1_BIG 0x40000000u
 
+// K = 8 needs 88 VGPRs unconstrained (5 waves per SIMD); asking for 6 waves
+// makes the allocator spill 68 bytes outside the hot loops instead
 template <int K, int T, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_sweep(const l2o_sweep_item *__restrict__ items,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >= 8 ? 6 : 1))) void k_sweep(const l2o_sweep_item *__restrict__ items,
                                                       l2o_inst *__restrict__ insts)
 {
     constexpr int UF = 4;  // steps per unrolled iteration

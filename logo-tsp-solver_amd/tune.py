@@ -7,7 +7,8 @@ use from its own directory:
   tuning_gfx950.txt        one instance in flight (logo2opt_create contexts,
                            engines with fewer than 8 live instances): each
                            shape is timed with HIP events (logo2opt_time_sweep)
-                           on a random and a nearest-neighbour tour
+                           on the nearest-neighbour start tour and on the
+                           local optimum a search reaches from it
   tuning_batch_gfx950.txt  batches (8+ live instances): each shape is timed on
                            an engine holding enough instances of that size to
                            fill the GPU (logo2opt_engine_time_sweep), and the
@@ -34,25 +35,43 @@ def pairs(n):
 
 
 def tune_single(reps=REPS, seed=1, verbose=True):
+    """Times every shape on the two ends of a local search from the greedy
+    nearest-neighbour start -- the start tour and the 2-opt local optimum the
+    search reaches (also what ILS kicks perturb) -- on uniform instances and,
+    from 16,000 cities, clustered ones."""
     best = []
+    default = L.tuning_get(L.TABLE_SINGLE)
     for n in reps:
         side = 10**4 if n <= 5000 else 10**6
-        xy = L.gen_uniform(n, side, seed + n)
-        tours = [L.random_perm(n, seed), L.nn_tour(xy, 0)]
-        ctx = L.Context(L.coords_of(xy))
-        sweeps = max(3, min(50, int(2e9 / (n * n))))
+        kinds = ["u", "c"] if n >= 16000 else ["u"]
+        cases = []
+        for kind in kinds:
+            xy = L.gen_uniform(n, side, seed + n) if kind == "u" else L.gen_clustered(n, side, seed + n)
+            ctx = L.Context(L.coords_of(xy))
+            r_nn = L.nn_tour(xy, 0)
+            tours = [r_nn]
+            if n <= 40000:  # the local optimum (a few seconds at most)
+                L.tuning_set(default, L.TABLE_SINGLE)
+                tours.append(ctx.optimize(r_nn, sync_every=64)["route"])
+            cases.append((ctx, tours))
+        sweeps = max(5, min(50, int(5e9 / (n * n))))
         times = {}
         for sh in L.tuning_shapes():
             L.tuning_set([(L.MAX_N, sh)], L.TABLE_SINGLE)
-            for t in tours:
-                ctx.time_sweep(t, 1)  # first launch of this shape
-            times[sh] = sum(ctx.time_sweep(t, sweeps) for t in tours) / len(tours)
-        ctx.close()
+            tot = 0.0
+            for ctx, tours in cases:
+                for t in tours:
+                    ctx.time_sweep(t, 1)  # first launch of this shape
+                    tot += ctx.time_sweep(t, sweeps)
+            times[sh] = tot
+        for ctx, _ in cases:
+            ctx.close()
         b = min(times, key=times.get)
         best.append((n, b))
         if verbose:
+            ncases = sum(len(tours) for _, tours in cases)
             ranked = sorted(times.items(), key=lambda kv: kv[1])[:4]
-            print(f"single n={n:>6}: " + "  ".join(f"{s}:{ms * 1e3:.1f}us" for s, ms in ranked), flush=True)
+            print(f"single n={n:>6}: " + "  ".join(f"{s}:{ms / ncases * 1e3:.1f}us" for s, ms in ranked), flush=True)
     return best
 
 
