@@ -72,27 +72,45 @@ def config2(seed=2):
 
 
 def cpu_baseline(n, xy, r0, budget_s):
-    """Bounded sample of the reference's CPU sweep on this host."""
+    """Bounded samples of the reference's CPU sweep on this host: its pthreads
+    CPUMTSolver::optimizeStep on up to 16 threads (the CPU share of one GPU
+    here) as the baseline, and its single-thread CPUSolver::optimizeStep."""
     ref_avx = os.path.join(ROOT, "oracle", "_ref", "libref2opt_avx.so")
     ref_sc = os.path.join(ROOT, "oracle", "_ref", "libref2opt.so")
     path = ref_avx if os.path.exists(ref_avx) else ref_sc if os.path.exists(ref_sc) else None
     fxy = np.ascontiguousarray(xy.astype(np.float32))
+    r0 = np.ascontiguousarray(r0, np.uint32)
     if path:
         R = C.CDLL(path)
-        R.ref_step.argtypes = [np.ctypeslib.ndpointer(np.float32), np.ctypeslib.ndpointer(np.uint32), C.c_uint32,
-                               np.ctypeslib.ndpointer(np.int32)]
+        f32, u32, i32 = (np.ctypeslib.ndpointer(t) for t in (np.float32, np.uint32, np.int32))
+        R.ref_step.argtypes = [f32, u32, C.c_uint32, i32]
         out = np.zeros(3, np.int32)
-        sweeps, t0 = 0, time.perf_counter()
-        while True:
-            R.ref_step(fxy, r0, n, out)
-            sweeps += 1
-            el = time.perf_counter() - t0
-            if el > budget_s or sweeps >= 50:
-                break
+
+        def sample(step, cap, budget):
+            sweeps, t0 = 0, time.perf_counter()
+            while True:
+                step()
+                sweeps += 1
+                el = time.perf_counter() - t0
+                if el > budget or sweeps >= cap:
+                    return sweeps, el
+
         variant = "AVX build (USE_CPU_SIMD, USE_AVX)" if path == ref_avx else "scalar build"
-        return {"value": sweeps * pairs(n) / el, "unit": "evals/s", "cores": 1, "kind": "reference",
-                "sample": f"{sweeps} CPUSolver::optimizeStep sweeps ({el:.1f} s) of the 33,810-city clustered "
-                          f"instance at its NN start tour, reference sources {variant}, one thread as --mode=cpu"}
+        s1, e1 = sample(lambda: R.ref_step(fxy, r0, n, out), 30, budget_s / 2)
+        single = {"value": s1 * pairs(n) / e1, "cores": 1,
+                  "sample": f"{s1} CPUSolver::optimizeStep sweeps ({e1:.1f} s), one thread as --mode=cpu"}
+        if hasattr(R, "ref_step_mt"):
+            R.ref_step_mt.argtypes = [f32, u32, C.c_uint32, C.c_int32, i32]
+            threads = min(16, len(os.sched_getaffinity(0)))
+            sm, em = sample(lambda: R.ref_step_mt(fxy, r0, n, threads, out), 200, budget_s / 2)
+            return {"value": sm * pairs(n) / em, "unit": "evals/s", "cores": threads, "kind": "reference",
+                    "sample": f"{sm} CPUMTSolver::optimizeStep sweeps ({em:.1f} s, {threads} pthreads as "
+                              f"--cores={threads}) of the 33,810-city clustered instance at its NN start tour, "
+                              f"reference sources {variant}",
+                    "single_thread": single}
+        single.update({"unit": "evals/s", "kind": "reference",
+                       "sample": single["sample"] + f" of the 33,810-city instance, reference sources {variant}"})
+        return single
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 

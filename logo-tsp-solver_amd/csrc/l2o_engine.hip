@@ -389,11 +389,14 @@ void drop_graph(logo2opt_engine *e)
     e->gsteps = 0;
 }
 
+// 1/L2O_TAIL_SPLIT of a launch's work, at its end, goes in quarter-height items
+#define L2O_TAIL_SPLIT 5
+
 // candidates (a, b), a+2 <= b <= n-2, of a sweep item (closed form)
 uint64_t item_work(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
 {
     const int64_t b0 = it.b0, a0 = it.a0;
-    const int64_t a_end = std::min<int64_t>(a0 + c.t, (int64_t)n - 3);  // rows a0 .. a_end-1
+    const int64_t a_end = std::min<int64_t>(a0 + it.rows, (int64_t)n - 3);  // rows a0 .. a_end-1
     const int64_t bhi = std::min<int64_t>(b0 + c.cols() - 1, (int64_t)n - 2);
     if (a_end <= a0 || bhi < b0) return 0;
     // rows a <= b0-2 see all of [b0, bhi]
@@ -427,7 +430,7 @@ int rebuild_lists(logo2opt_engine *e)
         // rows a <= n-4; the first tile of a row block starts at the
         // diagonal, b0 = a0 + 2
         for (uint32_t a0 = 0; a0 + 4 <= n; a0 += c.t)
-            for (uint32_t b0 = a0 + 2; b0 + 2 <= n; b0 += c.cols()) v.push_back({id, a0, b0, 0});
+            for (uint32_t b0 = a0 + 2; b0 + 2 <= n; b0 += c.cols()) v.push_back({id, a0, b0, (uint32_t)c.t});
         const uint32_t nch = (n + L2O_APPLY_CHUNK - 1) / L2O_APPLY_CHUNK;
         for (uint32_t ch = 0; ch < nch; ++ch) ai.push_back({id, ch});
         if (s.job.ils_rounds) {
@@ -452,8 +455,22 @@ int rebuild_lists(logo2opt_engine *e)
         std::vector<uint32_t> idx(v.size());
         for (uint32_t q = 0; q < idx.size(); ++q) idx[q] = q;
         std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
-        std::vector<l2o_sweep_item> sorted(v.size());
-        for (size_t q = 0; q < idx.size(); ++q) sorted[q] = v[idx[q]];
+        std::vector<l2o_sweep_item> sorted;
+        sorted.reserve(v.size() + v.size() / 2);
+        // The last L2O_TAIL_SPLIT of the work (the end of the dispatch) is
+        // issued as items of t/4 rows, so the launch drains in short pieces.
+        uint64_t total = 0, done = 0;
+        for (uint64_t w : work) total += w;
+        for (size_t q = 0; q < idx.size(); ++q) {
+            const l2o_sweep_item it = v[idx[q]];
+            if (c.t >= 16 && done >= total - total / L2O_TAIL_SPLIT) {
+                const uint32_t r4 = (uint32_t)c.t / 4;
+                for (uint32_t r = 0; r < it.rows; r += r4) sorted.push_back({it.inst, it.a0 + r, it.b0, r4});
+            } else {
+                sorted.push_back(it);
+            }
+            done += work[idx[q]];
+        }
         v.swap(sorted);
         const uint32_t cnt = (uint32_t)v.size();
         if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     const float slack = ci->slack;
     l2o_pt *const p0 = ci->pts[0], *const p1 = ci->pts[1];
     // (an empty asm that consumes them keeps the loads above the branches)
-    asm volatile("" ::"s"(phase), "s"(par), "s"(n), "s"(slack), "s"(p0), "s"(p1), "s"(it.a0), "s"(it.b0));
+    asm volatile("" ::"s"(phase), "s"(par), "s"(n), "s"(slack), "s"(p0), "s"(p1), "s"(it.a0), "s"(it.b0),
+                 "s"(it.rows));
     if (phase != L2O_LO) return;
     const gpt *__restrict__ P = (const gpt *)(par ? p1 : p0);
     const int a0 = (int)it.a0;
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     if (bw > n - 2) return;                   // wave-uniform: no candidate column
     const int bl = bw + lane * K;
     // rows a <= n-4 have candidates
-    int t_end = min(T, n - 3 - a0);
+    int t_end = min((int)it.rows, n - 3 - a0);
     // a wave whose first column reaches past the last row + 2 never meets the
     // diagonal b = a+1; otherwise rows past its last column - 2 are empty
     const bool diag = bw < a0 + t_end + 1;

@@ -70,13 +70,14 @@ struct l2o_inst {
     unsigned long long rng[4];
 };
 
-// One workgroup of the sweep kernel: rows a in [a0, a0 + T) and columns b in
-// [b0, b0 + 64 * waves * k) of the candidate triangle a + 2 <= b <= n - 2.
+// One workgroup of the sweep kernel: rows a in [a0, a0 + rows) and columns b
+// in [b0, b0 + 64 * waves * k) of the candidate triangle a + 2 <= b <= n - 2;
+// rows <= T of the launch's shape (the tail of a launch uses shorter items).
 struct l2o_sweep_item {
     uint32_t inst;
     uint32_t a0;
     uint32_t b0;
-    uint32_t pad;
+    uint32_t rows;
 };
 
 // One workgroup of the apply kernel: elements [chunk*CHUNK, (chunk+1)*CHUNK).

@@ -13,6 +13,9 @@
 #include <headers.h>
 #include <cpuSolver.h>
 #include <ILSGlobalSolver.h>
+#ifdef L2O_REF_MT
+#include <cpuMTSolver.h>
+#endif
 
 #include <cstdint>
 #include <cstdlib>
@@ -90,6 +93,30 @@ uint64_t ref_optimize(const float* xy, uint32_t* route, uint32_t n, int32_t* log
     }
     return s.log.size();
 }
+
+#ifdef L2O_REF_MT
+// CPUMTSolver::optimizeStep is protected (cpuMTSolver.h)
+class MTProbe : public CPUMTSolver {
+public:
+    MTProbe(city_coords* c, cmdArguments* a, ROUTE_DATA_TYPE n) : CPUMTSolver(c, a, std::string("ref-mt"), 0, n) {}
+    struct best2_out step(const vector<ROUTE_DATA_TYPE>& r) { return CPUMTSolver::optimizeStep(r); }
+};
+
+// one CPUMTSolver::optimizeStep (cpuMTSolver.cpp:75-158) over `threads` pthreads
+// (the reference's --cores); out = {i, j, minchange}.  Built only into the
+// SIMD library, where it is the timed multi-core CPU baseline.
+void ref_step_mt(const float* xy, const uint32_t* route, uint32_t n, int32_t threads, int32_t* out) {
+    cmdArguments a = quietArgs();
+    a.maxCoresUsed = threads;
+    a.setAffinity = 0;
+    MTProbe s((city_coords*)xy, &a, n);
+    std::vector<ROUTE_DATA_TYPE> r(route, route + n);
+    struct best2_out o = s.step(r);
+    out[0] = (int32_t)o.i;
+    out[1] = (int32_t)o.j;
+    out[2] = o.minchange;
+}
+#endif
 
 // srand(seed) then one reference kick
 void ref_kick(uint32_t* route, uint32_t n, unsigned seed) {
