@@ -344,16 +344,18 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
         int t = 0;
         const int t_main = t_end - t_end % UF;
         if (t_main > 0) {
-            int cur[UF][3];
-            load_rows(0, cur, UFc{});
-            for (; t < t_main; t += UF) {
-                int nxt[UF][3];
-                load_rows(min(t + UF, t_main - UF), nxt, UFc{});
-                if (block(t, cur, UFc{}, mask_tag)) slow(t, UF);
-#pragma unroll
-                for (int s = 0; s < UF; ++s)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) cur[s][q] = nxt[s][q];
+            // two row buffers used in turn (no register copies between blocks)
+            int ra[UF][3], rb[UF][3];
+            load_rows(0, ra, UFc{});
+            for (; t < t_main; t += 2 * UF) {
+                load_rows(min(t + UF, t_main - UF), rb, UFc{});
+                if (block(t, ra, UFc{}, mask_tag)) slow(t, UF);
+                if (t + UF >= t_main) {
+                    t += UF;
+                    break;
+                }
+                load_rows(min(t + 2 * UF, t_main - UF), ra, UFc{});
+                if (block(t + UF, rb, UFc{}, mask_tag)) slow(t + UF, UF);
             }
         }
         for (; t < t_end; ++t) {
