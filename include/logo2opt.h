@@ -15,9 +15,10 @@
  *     (src/defs.h:181-187), logo2opt_process_time is process_time
  *     (src/defs.h:190-198).
  *   - Coordinates must be integral with x and y spans below 2^22 (they are
- *     translated so the minimum is 0); edge lengths are the exact TSPLIB
- *     EUC_2D nint(sqrt(dx^2+dy^2)).  Tours are permutations of 0..n-1,
- *     3 <= n <= 2^20.
+ *     translated so the minimum is 0); edge lengths are exact TSPLIB lengths:
+ *     EUC_2D nint(sqrt(dx^2+dy^2)) by default (the reference's metric), or
+ *     CEIL_2D / ATT chosen per context or job (LOGO2OPT_METRIC_*).  Tours are
+ *     permutations of 0..n-1, 3 <= n <= 2^20.
  *   - A move (i,J) names tour positions, 1 <= i, i+2 <= J <= n-1.  It is
  *     applied as rotate+reverse: r' = r[i..J-1] ++ r[i-1..0] ++ r[n-1..J]
  *     (src/solvers/twoOptLocalSolver.cpp:41-48).  Among the minimal gains the
@@ -58,11 +59,24 @@ typedef struct {                                                      /* defs.h:
 #define LOGO2OPT_EAGAIN     -6  /* engine full: HBM budget or slot limit       */
 #define LOGO2OPT_ESTATE     -7  /* call not valid in the current state         */
 #define LOGO2OPT_ENOTFOUND  -8  /* unknown slot / request id                   */
+#define LOGO2OPT_EFORMAT    -9  /* not a supported TSPLIB .tsp / .tour file    */
+#define LOGO2OPT_EIO       -10  /* file could not be opened, read or written   */
 const char *logo2opt_strerror(int code);
 
 #define LOGO2OPT_MAX_N (1u << 20)
 #define LOGO2OPT_KICK_DOUBLE_BRIDGE 0 /* build-defined seeded double-bridge        */
 #define LOGO2OPT_KICK_REFERENCE     1 /* rotate+reverse, ILSGlobalSolver.cpp:37-48 */
+
+/* Edge lengths (TSPLIB EDGE_WEIGHT_TYPE) for s = dx^2 + dy^2:
+ *   EUC_2D  nint(sqrt(s))      the reference's metric (headers.h:126-142,
+ *                              exact here)
+ *   CEIL_2D ceil(sqrt(s))      e.g. pla7397, pla33810, pla85900
+ *   ATT     ceil(sqrt(s/10))   TSPLIB's pseudo-Euclidean distance (att532)
+ * The reference reads every file as EUC_2D (tsp_loader.c:49-162 ignores
+ * EDGE_WEIGHT_TYPE); CEIL_2D and ATT have no counterpart there. */
+#define LOGO2OPT_METRIC_EUC_2D  0
+#define LOGO2OPT_METRIC_CEIL_2D 1
+#define LOGO2OPT_METRIC_ATT     2
 
 /* ======================================================================== */
 /* Single-instance context: the CUDASolver replacement                      */
@@ -73,6 +87,10 @@ typedef struct logo2opt_ctx logo2opt_ctx;
  * resident buffers and copy the coordinates once. */
 int logo2opt_create(int device, uint32_t n, const logo2opt_city_coords *coords,
                     logo2opt_ctx **out);
+/* logo2opt_create with the edge-length metric (LOGO2OPT_METRIC_*) that every
+ * call on the context uses; logo2opt_create is the EUC_2D case. */
+int logo2opt_create_metric(int device, uint32_t n, const logo2opt_city_coords *coords,
+                           int metric, logo2opt_ctx **out);
 /* CUDASolver::close / cleanCUDA (CUDASolver.cpp:141-144,388-401). */
 int logo2opt_destroy(logo2opt_ctx *ctx);
 
@@ -145,7 +163,7 @@ typedef struct {
     uint32_t kick;         /* LOGO2OPT_KICK_*                                     */
     uint64_t seed;         /* kick generator seed                                 */
     uint32_t log_cap;      /* moves to log (0 = none)                             */
-    uint32_t reserved;
+    uint32_t metric;       /* LOGO2OPT_METRIC_* (0 = EUC_2D)                      */
 } logo2opt_job;
 
 typedef struct {
@@ -270,6 +288,63 @@ void logo2opt_random_perm(uint32_t n, uint64_t seed, uint32_t *route);
 int  logo2opt_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route);
 /* int32 xy -> reference city_coords */
 void logo2opt_xy_to_coords(uint32_t n, const int32_t *xy, logo2opt_city_coords *coords);
+
+/* ======================================================================== */
+/* TSPLIB files (host)                                                       */
+/* ======================================================================== */
+typedef struct {
+    char     name[64];     /* NAME (truncated)                                    */
+    uint32_t n;            /* DIMENSION                                           */
+    int32_t  metric;       /* LOGO2OPT_METRIC_* from EDGE_WEIGHT_TYPE             */
+    double   offset_x;     /* subtracted from every x (the minimum, rounded when  */
+    double   offset_y;     /*   LOGO2OPT_TSP_ROUND is given), and from every y    */
+    uint32_t integral;     /* 1 when every coordinate in the file is an integer  */
+    uint32_t reserved;
+} logo2opt_tsp_info;
+
+/* Round non-integral coordinates to the nearest integer (halves away from
+ * zero) instead of failing with LOGO2OPT_ECOORD; lengths are then those of
+ * the rounded instance. */
+#define LOGO2OPT_TSP_ROUND 1u
+
+/* readFileCoords (src/utils/tsp_loader.c:49-162): reads a TSPLIB .tsp file.
+ * The reference skips a fixed number of header lines and always uses EUC_2D;
+ * here the header is parsed by keyword in any order ("KEY : value", spaces
+ * optional), TYPE must be TSP, EDGE_WEIGHT_TYPE EUC_2D, CEIL_2D or ATT, and
+ * NODE_COORD_SECTION must list every id 1..DIMENSION once (any order; decimal
+ * or exponent notation).  info is filled whenever the header parses.  When
+ * coords is not NULL it receives the n coordinates (by id - 1) translated by
+ * (offset_x, offset_y) so the minimum is 0, which leaves every edge length of
+ * the three metrics unchanged; cap must be >= n.  Errors: LOGO2OPT_EIO (open
+ * or read), LOGO2OPT_EFORMAT (not TSPLIB, unsupported TYPE /
+ * EDGE_WEIGHT_TYPE / NODE_COORD_TYPE, bad or missing entries),
+ * LOGO2OPT_ECOORD (non-integral coordinates without LOGO2OPT_TSP_ROUND, or a
+ * span >= 2^22), LOGO2OPT_EINVAL (null path/info, cap < n with coords). */
+int logo2opt_tsp_read(const char *path, uint32_t flags, logo2opt_tsp_info *info,
+                      logo2opt_city_coords *coords, uint32_t cap);
+
+/* routeInitFromFile (src/utils/common.c:109-129): reads a tour of n cities
+ * into route (0-based ids).  Accepts the TSPLIB form (header, TOUR_SECTION,
+ * 1-based ids, terminated by -1 or EOF; a DIMENSION must equal n) and the
+ * reference's own header-less form (n 1-based ids, one per line, as
+ * saveRouteToAFile writes them).  Errors: LOGO2OPT_EIO, LOGO2OPT_EFORMAT
+ * (wrong count, non-numeric entries, TYPE other than TOUR),
+ * LOGO2OPT_ETOUR (not a permutation). */
+int logo2opt_tour_read(const char *path, uint32_t n, uint32_t *route);
+
+/* saveRouteToAFile (src/utils/common.c:153-192): writes route (0-based ids)
+ * as a TSPLIB tour: NAME, COMMENT "length <length>" (when length > 0),
+ * TYPE : TOUR, DIMENSION, TOUR_SECTION, the 1-based ids, -1, EOF.  With
+ * headerless != 0 only the 1-based ids are written, one per line, like the
+ * reference.  The caller chooses the path (the reference appends
+ * "_<length>.tour" itself). */
+int logo2opt_tour_write(const char *path, const char *name, uint32_t n, const uint32_t *route,
+                        uint64_t length, int headerless);
+
+/* Length of route under a metric on the host (exact, uint64); the same
+ * function as the device's.  For checking files and tours without a GPU. */
+int logo2opt_host_route_length(uint32_t n, const logo2opt_city_coords *coords, int metric,
+                               const uint32_t *route, uint64_t *length);
 
 #ifdef __cplusplus
 }

@@ -3,8 +3,9 @@
     python logo-tsp-solver_amd/build.py
 
 The library is the product: hand-written gfx950 kernels (csrc/l2o_kernels.hip),
-the host engine behind include/logo2opt.h (csrc/l2o_engine.hip) and the
-workload generators (csrc/l2o_workload.cpp).  It links only the HIP runtime.
+the host engine behind include/logo2opt.h (csrc/l2o_engine.hip), the
+workload generators (csrc/l2o_workload.cpp), the server (csrc/l2o_server.cpp)
+and the TSPLIB file reader/writer (csrc/l2o_tsplib.cpp).  It links only the HIP runtime.
 """
 import os
 import shutil
@@ -14,7 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "liblogo2opt.so")
-SOURCES = ["csrc/l2o_kernels.hip", "csrc/l2o_engine.hip", "csrc/l2o_workload.cpp", "csrc/l2o_server.cpp"]
+SOURCES = ["csrc/l2o_kernels.hip", "csrc/l2o_engine.hip", "csrc/l2o_workload.cpp", "csrc/l2o_server.cpp",
+           "csrc/l2o_tsplib.cpp"]
 ARCH = os.environ.get("LOGO2OPT_ARCH", "gfx950")
 
 

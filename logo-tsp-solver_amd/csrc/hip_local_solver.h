@@ -11,6 +11,11 @@
 //   TwoOptLocalSolver::optimize            optimize    -> logo2opt_optimize
 //     (twoOptLocalSolver.cpp:22-64; host apply per move)   (apply on the device)
 //
+//   reference file helpers                 free functions (this file)
+//   readFileCoords  tsp_loader.c:49-162    readFileCoords    -> logo2opt_tsp_read
+//   routeInitFromFile common.c:109-129     routeInitFromFile -> logo2opt_tour_read
+//   saveRouteToAFile common.c:153-192      saveRouteToAFile  -> logo2opt_tour_write
+//
 // Errors: the reference calls exit(-1) (UtilsCUDA.cpp:318-328); the adapter
 // throws std::runtime_error carrying logo2opt_strerror().
 #pragma once
@@ -25,8 +30,10 @@ namespace logo2opt {
 
 class HIPSolver {
 public:
-    HIPSolver(const logo2opt_city_coords *coords, uint32_t size, int deviceId = 0, uint32_t syncEvery = 64)
-        : coords_(coords), size_(size), device_(deviceId), sync_every_(syncEvery)
+    // metric: LOGO2OPT_METRIC_EUC_2D (the reference's), _CEIL_2D or _ATT
+    HIPSolver(const logo2opt_city_coords *coords, uint32_t size, int deviceId = 0, uint32_t syncEvery = 64,
+              int metric = LOGO2OPT_METRIC_EUC_2D)
+        : coords_(coords), size_(size), device_(deviceId), sync_every_(syncEvery), metric_(metric)
     {
         init();
     }
@@ -36,7 +43,7 @@ public:
 
     void init()
     {
-        if (!ctx_) check(logo2opt_create(device_, size_, coords_, &ctx_), "logo2opt_create");
+        if (!ctx_) check(logo2opt_create_metric(device_, size_, coords_, metric_, &ctx_), "logo2opt_create_metric");
     }
 
     void close()
@@ -100,7 +107,46 @@ private:
     uint32_t size_;
     int device_;
     uint32_t sync_every_;
+    int metric_;
     logo2opt_ctx *ctx_ = nullptr;
 };
+
+namespace detail {
+inline void check(int rc, const std::string &what)
+{
+    if (rc != LOGO2OPT_OK) throw std::runtime_error(what + ": " + logo2opt_strerror(rc));
+}
+}  // namespace detail
+
+// readFileCoords (tsp_loader.c:49-162): the coordinates of a TSPLIB .tsp file
+// (translated so the minimum is 0), with its EDGE_WEIGHT_TYPE in *metric.
+inline std::vector<logo2opt_city_coords> readFileCoords(const char *filename, int *metric = nullptr,
+                                                        uint32_t flags = 0)
+{
+    logo2opt_tsp_info info;
+    detail::check(logo2opt_tsp_read(filename, flags, &info, nullptr, 0), std::string("read ") + filename);
+    std::vector<logo2opt_city_coords> c(info.n);
+    detail::check(logo2opt_tsp_read(filename, flags, &info, c.data(), info.n), std::string("read ") + filename);
+    if (metric) *metric = info.metric;
+    return c;
+}
+
+// routeInitFromFile (common.c:109-129): a tour of `size` cities, TSPLIB or
+// header-less form, appended to route as 0-based ids.
+inline void routeInitFromFile(std::vector<uint32_t> &route, uint32_t size, const char *filename)
+{
+    std::vector<uint32_t> r(size);
+    detail::check(logo2opt_tour_read(filename, size, r.data()), std::string("read ") + filename);
+    route.insert(route.end(), r.begin(), r.end());
+}
+
+// saveRouteToAFile (common.c:153-165): writes <filename>_<length>.tour in the
+// reference's header-less layout (1-based ids, one per line).
+inline void saveRouteToAFile(const std::vector<uint32_t> &route, const char *filename, unsigned long length)
+{
+    const std::string fn = std::string(filename) + "_" + std::to_string(length) + ".tour";
+    detail::check(logo2opt_tour_write(fn.c_str(), filename, (uint32_t)route.size(), route.data(), length, 1),
+                  "write " + fn);
+}
 
 }  // namespace logo2opt

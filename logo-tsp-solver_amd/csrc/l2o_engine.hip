@@ -181,19 +181,11 @@ struct arena {
     }
 };
 
-// exact nint(sqrt(s)) on the host
-inline uint32_t nint_sqrt_u64(uint64_t s)
-{
-    uint64_t k = (uint64_t)std::sqrt((double)s);
-    while (k * k > s) --k;
-    while ((k + 1) * (k + 1) <= s) ++k;
-    return (uint32_t)(k + (s > k * k + k ? 1 : 0));
-}
-
-inline uint32_t dist_i(const int32_t *xy, uint32_t a, uint32_t b)
+// exact metric length of the edge a -> b on the host (same function as the kernels)
+inline uint32_t dist_i(const int32_t *xy, uint32_t a, uint32_t b, uint32_t metric)
 {
     const int64_t dx = (int64_t)xy[2 * a] - xy[2 * b], dy = (int64_t)xy[2 * a + 1] - xy[2 * b + 1];
-    return nint_sqrt_u64((uint64_t)(dx * dx + dy * dy));
+    return (uint32_t)l2o::metric_dist((long long)(dx * dx + dy * dy), metric);
 }
 
 uint64_t pairs_of(uint32_t n) { return n >= 4 ? (uint64_t)(n - 2) * (uint64_t)(n - 3) / 2 : 0; }
@@ -262,6 +254,7 @@ struct logo2opt_engine {
 struct logo2opt_ctx {
     logo2opt_engine *e = nullptr;
     uint32_t n = 0;
+    uint32_t metric = LOGO2OPT_METRIC_EUC_2D;
     std::vector<logo2opt_city_coords> coords;
     int32_t slot = -1;
     uint64_t sweeps = 0, moves = 0, evals = 0, fallbacks = 0;
@@ -290,11 +283,13 @@ int check_coords(uint32_t n, const logo2opt_city_coords *c, std::vector<int32_t>
         xy[2 * k] = (int32_t)(c[k].x - mnx);
         xy[2 * k + 1] = (int32_t)(c[k].y - mny);
     }
-    // |exact gain - FP32 estimate| <= 1 (two nint roundings) + 2 e_d + rounding
-    // of the three FP32 adds and of the threshold, with e_d <= dmax * 2^-21
-    // (covers FP32 rounding of s and up to 3.5 ulp of v_sqrt_f32; tests check
-    // v_sqrt_f32 against the correctly rounded sqrt) and each add within
-    // ulp(4 dmax) <= dmax * 2^-21.  Total <= 1 + dmax * 2^-18; 2^-8 margin.
+    // |exact gain - FP32 estimate| <= 1 (|D - (sqrt(s) scale + add)| <= 1/2 per
+    // new edge for every metric) + 2 e_d + rounding of the three FP32 adds and
+    // of the threshold, with e_d <= dmax * 2^-21 (covers FP32 rounding of s,
+    // up to 3.5 ulp of v_sqrt_f32 -- tests check v_sqrt_f32 against the
+    // correctly rounded sqrt -- and for ATT the rounding of 1/sqrt(10)), each
+    // add within ulp(4 dmax) <= dmax * 2^-21 and each estimate's fma within
+    // dmax * 2^-23.  Total <= 1 + dmax * 2^-18; 2^-8 margin.
     const double dmax = std::sqrt((mxx - mnx) * (mxx - mnx) + (mxy - mny) * (mxy - mny));
     const double s = 1.0 + dmax * std::ldexp(1.0, -18) + std::ldexp(1.0, -8);
     *slack = std::nextafter((float)s, INFINITY);
@@ -350,6 +345,9 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
     in.slack = s.slack;
     in.ils_rounds = s.job.ils_rounds;
     in.kick = s.job.kick;
+    in.metric = s.job.metric;
+    in.est_scale = s.job.metric == l2o::L2O_ATT ? (float)(1.0 / std::sqrt(10.0)) : 1.0f;
+    in.est_add = s.job.metric == l2o::L2O_EUC_2D ? 0.0f : 0.5f;
     in.phase = phase;
     in.key = L2O_SENTINEL;
     // SplitMix64 -> xoshiro256** state (same seeding as the workload generators)
@@ -365,12 +363,12 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
     const int32_t *xy = s.xy.data();
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t c = route[k], pc = route[k == 0 ? n - 1 : k - 1];
-        const uint32_t d = dist_i(xy, pc, c);
+        const uint32_t d = dist_i(xy, pc, c, s.job.metric);
         L += d;
         pts[k].x = xy[2 * c];
         pts[k].y = xy[2 * c + 1];
         pts[k].d = (int32_t)d;
-        pts[k].w = l2o::ceil_sqrt2((int32_t)d);
+        pts[k].w = l2o::ceil_sqrtg((int32_t)d, l2o::tier1_g(s.job.metric));
     }
     in.length = L;
     L2O_TRY(hipMemcpyAsync(in.pts[0], pts.data(), 16ull * n, hipMemcpyHostToDevice, e->stream));
@@ -638,7 +636,7 @@ int engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coord
     if (id < 0) return LOGO2OPT_EAGAIN;
     logo2opt_job j{};
     if (job) j = *job;
-    if (j.kick > 1) return LOGO2OPT_EINVAL;
+    if (j.kick > 1 || j.metric > LOGO2OPT_METRIC_ATT) return LOGO2OPT_EINVAL;
     if (j.ils_rounds && n < 8) return LOGO2OPT_EINVAL;  // double-bridge needs room for 3 cuts
     const uint64_t bytes = bytes_for(n, &j);
     uint64_t off;
@@ -774,6 +772,8 @@ const char *logo2opt_strerror(int code)
     case LOGO2OPT_EAGAIN: return "engine full (HBM budget or slots)";
     case LOGO2OPT_ESTATE: return "invalid state for this call";
     case LOGO2OPT_ENOTFOUND: return "unknown slot";
+    case LOGO2OPT_EFORMAT: return "not a supported TSPLIB file";
+    case LOGO2OPT_EIO: return "file could not be opened, read or written";
     default: return "unknown error";
     }
 }
@@ -954,7 +954,14 @@ int logo2opt_engine_set_graphs(logo2opt_engine *e, int on)
 // ---------------------------------------------------------------------------
 int logo2opt_create(int device, uint32_t n, const logo2opt_city_coords *coords, logo2opt_ctx **out)
 {
+    return logo2opt_create_metric(device, n, coords, LOGO2OPT_METRIC_EUC_2D, out);
+}
+
+int logo2opt_create_metric(int device, uint32_t n, const logo2opt_city_coords *coords, int metric,
+                           logo2opt_ctx **out)
+{
     if (!out || !coords || n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
+    if (metric < LOGO2OPT_METRIC_EUC_2D || metric > LOGO2OPT_METRIC_ATT) return LOGO2OPT_EINVAL;
     std::vector<int32_t> xy;
     float slack;
     int rc = check_coords(n, coords, xy, &slack);
@@ -962,6 +969,7 @@ int logo2opt_create(int device, uint32_t n, const logo2opt_city_coords *coords, 
     logo2opt_ctx *c = new (std::nothrow) logo2opt_ctx();
     if (!c) return LOGO2OPT_ENOMEM;
     c->n = n;
+    c->metric = (uint32_t)metric;
     c->coords.assign(coords, coords + n);
     // room for one ILS instance with a modest move log
     logo2opt_job j{};
@@ -998,8 +1006,9 @@ void ctx_accumulate(logo2opt_ctx *c)
     }
 }
 
-int ctx_load(logo2opt_ctx *c, const uint32_t *route, const logo2opt_job &job)
+int ctx_load(logo2opt_ctx *c, const uint32_t *route, logo2opt_job job)
 {
+    job.metric = c->metric;
     if (c->slot >= 0) {
         ctx_accumulate(c);
         int rc = engine_remove(c->e, c->slot);

@@ -7,26 +7,29 @@
 //
 // Candidates.  With a = i-1 and b = J-1 a candidate is a pair
 // 0 <= a, a+2 <= b <= n-2 with exact gain
-//     G = n(a,b) + n(a+1,b+1) - D[a+1] - D[b+1],   n(x,y) = nint(d(P[x],P[y])),
+//     G = n(a,b) + n(a+1,b+1) - D[a+1] - D[b+1],
+// n(x,y) the exact metric length of P[x] -> P[y] (TSPLIB EUC_2D nint(d), or
+// CEIL_2D / ATT, per instance),
 // D[k] = n(k-1,k)  (cpuSolver.cpp:133-138 in tour-position form).  The sweep
 // returns the lexicographic minimum of (G, i, J) over G < 0, the move the
 // reference's strict-'<' scan keeps.
 //
 // Three filter tiers make every candidate cost four VALU operations:
-//  1. Integer L1 bound.  nint(d) >= d - 1/2 and |dx|+|dy| <= sqrt(2) d, so
-//     G <= best-1 implies
-//         L1(a,b) + L1(a+1,b+1) <= ceil(sqrt2*best) + W[a+1] + W[b+1],
-//     W[k] = ceil(sqrt(2) D[k]) (stored with each point).  All terms are
+//  1. Integer L1 bound.  |dx|+|dy| <= sqrt(2) d and d < D + 1/2 (EUC_2D;
+//     for CEIL_2D d <= D, for ATT d <= sqrt(10) D), so G <= best-1 implies
+//         L1(a,b) + L1(a+1,b+1) <= ceil(sqrt(g)*best) + W[a+1] + W[b+1],
+//     W[k] = ceil(sqrt(g) D[k]) (stored with each point), g = 2 (EUC_2D,
+//     CEIL_2D) or 20 (ATT); see tier1_g in l2o_launch.h.  All terms are
 //     integers: the test is exact, no rounding margin.  A lane owns K+1
 //     consecutive columns b and keeps per-column offsets o with
 //     o_k + o_{k+1} = -W[b_k + 1]; then V(a,k) = L1(a, b_k) + o_k is two
 //     v_sad_u32 (|x-x'| + acc) and the test of candidate (a, b_k) is
-//     V(a,k) + V(a+1,k+1) <= ceil(sqrt2*best) + W[a+1], whose right side is
+//     V(a,k) + V(a+1,k+1) <= ceil(sqrt(g)*best) + W[a+1], whose right side is
 //     wave-uniform.  Each V is computed once and used by two candidates.
 //  2. FP32 estimate q(a,b) + q(a+1,b+1) - D[a+1] - D[b+1] <= best-1+slack,
 //     |exact - estimate| <= slack (per instance, set by the host from the
 //     coordinate span), only for lanes flagged by tier 1.
-//  3. Exact FP64 sqrt + integer fix-up for the survivors.
+//  3. Exact FP64 sqrt + integer fix-up for the survivors (metric_dist).
 // Each tier only drops candidates with G > cap, where cap is the smaller of
 // (the wave's best exact gain of EARLIER steps) - 1 -- a later step has a
 // larger i, so a tie with that best can never win -- and the best gain
@@ -49,26 +52,25 @@
 namespace l2o {
 
 // ---------------------------------------------------------------------------
-// exact TSPLIB EUC_2D distance: nint(sqrt(s)) = k + [s > k^2 + k], k = isqrt(s)
+// exact metric length of the edge P1 -> P2 (metric_dist in l2o_launch.h)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int exact_dist(int x1, int y1, int x2, int y2)
+__device__ __forceinline__ int exact_dist(int x1, int y1, int x2, int y2, uint32_t metric)
 {
     // coordinates lie in [0, 2^22): the differences and s are exact
     const long long dx = (long long)(x1 - x2);
     const long long dy = (long long)(y1 - y2);
-    const long long s = dx * dx + dy * dy;
-    long long k = (long long)sqrt((double)s);
-    if (k * k > s) k -= 1;
-    else if ((k + 1) * (k + 1) <= s) k += 1;
-    return (int)(k + (s > k * k + k ? 1 : 0));
+    return metric_dist(dx * dx + dy * dy, metric);
 }
 
-// FP32 estimate of d(P1,P2): the second filter tier (error bounded by slack).
-// The integer differences convert to FP32 exactly.
-__device__ __forceinline__ float est_dist(int x1, int y1, int x2, int y2)
+// FP32 estimate of the metric length: sqrt(s) * scale + add, with scale = 1,
+// add = 0 for EUC_2D (D = nint(d)), scale = 1, add = 1/2 for CEIL_2D (D in
+// [d, d+1)) and scale = 1/sqrt(10), add = 1/2 for ATT; |D - estimate| <= 1/2
+// plus the FP32 error, which the instance's slack bounds.  The integer
+// differences convert to FP32 exactly.
+__device__ __forceinline__ float est_dist(int x1, int y1, int x2, int y2, float scale, float add)
 {
     const float dx = (float)(x1 - x2), dy = (float)(y1 - y2);
-    return __builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+    return __builtin_fmaf(__builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, dy * dy)), scale, add);
 }
 
 // |a - b| + c: one v_sad_u32
@@ -77,10 +79,10 @@ __device__ __forceinline__ uint32_t sad(uint32_t a, uint32_t b, uint32_t c)
     return (a > b ? a - b : b - a) + c;
 }
 
-// ceil(sqrt(2) * best) for best <= 0
-__device__ __forceinline__ int tier1_base(int best)
+// ceil(sqrt(g) * best) for best <= 0 (sqrt(g) * best is irrational unless 0)
+__device__ __forceinline__ int tier1_base(int best, int g)
 {
-    return best == 0 ? 0 : 1 - ceil_sqrt2(-best);
+    return best == 0 ? 0 : 1 - ceil_sqrtg(-best, g);
 }
 
 __device__ __forceinline__ unsigned long long make_key(int gain, uint32_t i, uint32_t j)
@@ -179,10 +181,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     const uint32_t phase = ci->phase, par = ci->parity;
     const int n = (int)ci->n;
     const float slack = ci->slack;
+    const uint32_t metric = ci->metric;
+    const float es = ci->est_scale, ea = ci->est_add;
     l2o_pt *const p0 = ci->pts[0], *const p1 = ci->pts[1];
     // (an empty asm that consumes them keeps the loads above the branches)
     asm volatile("" ::"s"(phase), "s"(par), "s"(n), "s"(slack), "s"(p0), "s"(p1), "s"(it.a0), "s"(it.b0),
-                 "s"(it.rows));
+                 "s"(it.rows), "s"(metric), "s"(es), "s"(ea));
     if (phase != L2O_LO) return;
     const gpt *__restrict__ P = (const gpt *)(par ? p1 : p0);
     const int a0 = (int)it.a0;
@@ -233,11 +237,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
         return uniform_u64(__hip_atomic_load(&in->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     };
     int cap = -1;
-    int tb = 0;                // tier 1: ceil(sqrt2 * (cap + 1))
+    const int g = tier1_g(metric);
+    int tb = 0;                // tier 1: ceil(sqrt(g) * (cap + 1))
     float thr = slack - 1.0f;  // tier 2: cap + slack
     auto set_cap = [&](int c) __attribute__((always_inline)) {
         cap = c;
-        tb = __builtin_amdgcn_readfirstlane(tier1_base(c + 1));
+        tb = __builtin_amdgcn_readfirstlane(tier1_base(c + 1, g));
         thr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)c + slack)));
     };
     {
@@ -265,12 +270,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
                 const uint32_t vb = sad((uint32_t)yb, cy[k + 1], sad((uint32_t)xb, cx[k + 1], o[k + 1]));
                 if ((int)(va + vb) <= u && (!diag || t - lim0 <= k)) {
                     const int dc = P[bl + k + 1].d;
-                    const float w2 = (est_dist(xa, ya, (int)cx[k], (int)cy[k]) +
-                                      est_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1])) - (float)dc;
+                    const float w2 = (est_dist(xa, ya, (int)cx[k], (int)cy[k], es, ea) +
+                                      est_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1], es, ea)) - (float)dc;
                     if (w2 <= u2) {
                         ++fb;
-                        const int gain = exact_dist(xa, ya, (int)cx[k], (int)cy[k]) +
-                                         exact_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1]) - db - dc;
+                        const int gain = exact_dist(xa, ya, (int)cx[k], (int)cy[k], metric) +
+                                         exact_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1], metric) - db -
+                                         dc;
                         if (gain < 0) {
                             const uint32_t i = (uint32_t)(a0 + t + 1);
                             const uint32_t j = (uint32_t)(bl + k + 1);
@@ -406,6 +412,7 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
     const uint32_t J = (uint32_t)key & 0xFFFFFu;
     const uint32_t L = J - i;
     const uint32_t par = in->parity;
+    const uint32_t metric = in->metric;
     const gpt *__restrict__ P = (const gpt *)in->pts[par];
     const gu32 *__restrict__ R = (const gu32 *)in->tour[par];
     gpt *__restrict__ Pn = (gpt *)in->pts[par ^ 1];
@@ -454,8 +461,8 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
         if (k == 0 || k == L) {
             // the two new edges: P[i] -> P[J] at position 0, P[J-1] -> P[i-1] at L
             const l2o_pt o = ld_pt(P, k == 0 ? J : J - 1);
-            r.d = exact_dist(r.x, r.y, o.x, o.y);
-            r.w = ceil_sqrt2(r.d);
+            r.d = exact_dist(r.x, r.y, o.x, o.y, metric);
+            r.w = ceil_sqrtg(r.d, tier1_g(metric));
         } else {
             r.d = q[e].d;
             r.w = q[e].w;
@@ -591,18 +598,20 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
     };
     l2o_pt *__restrict__ Pn = in->pts[0];
     uint32_t *__restrict__ Rn = in->tour[0];
+    const uint32_t metric = in->metric;
+    const int g = tier1_g(metric);
     unsigned long long sum = 0;
     for (uint32_t k = threadIdx.x; k < n; k += L2O_ILS_THREADS) {
         const uint32_t s = src_of(k);
         const uint32_t sp = src_of(k == 0 ? n - 1 : k - 1);
         const l2o_pt p = BP[s], q = BP[sp];
-        const int d = exact_dist(q.x, q.y, p.x, p.y);
+        const int d = exact_dist(q.x, q.y, p.x, p.y, metric);
         sum += (unsigned long long)d;
         l2o_pt r;
         r.x = p.x;
         r.y = p.y;
         r.d = d;
-        r.w = ceil_sqrt2(d);
+        r.w = ceil_sqrtg(d, g);
         Pn[k] = r;
         Rn[k] = BR[s];
     }
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(256) void k_length(const l2o_inst *__restrict__ ins
     unsigned long long sum = 0;
     for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
         const l2o_pt p = P[k], q = P[k == 0 ? n - 1 : k - 1];
-        sum += (unsigned long long)exact_dist(q.x, q.y, p.x, p.y);
+        sum += (unsigned long long)exact_dist(q.x, q.y, p.x, p.y, in->metric);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

@@ -6,17 +6,52 @@
 
 namespace l2o {
 
-// ceil(sqrt(2) * d) for 0 <= d < 2^24.  For d >= 1, sqrt(2) d is irrational,
-// so the ceiling is isqrt(2 d^2) + 1; isqrt is exact after the correction.
-__host__ __device__ inline int32_t ceil_sqrt2(int32_t d)
+// Edge-length metrics (TSPLIB EDGE_WEIGHT_TYPE), for integer s = dx^2 + dy^2:
+//   EUC_2D  nint(sqrt(s))
+//   CEIL_2D ceil(sqrt(s))
+//   ATT     ceil(sqrt(s / 10)): TSPLIB's t = nint(r), r = sqrt(s/10), plus 1
+//           when t < r, equals ceil(r) for every s (when r is not an integer,
+//           nint(r) < r exactly when the fraction of r is below 1/2)
+enum : uint32_t { L2O_EUC_2D = 0, L2O_CEIL_2D = 1, L2O_ATT = 2 };
+
+// Exact metric length.  With f = 10 for ATT and 1 otherwise,
+// m = floor(sqrt(s / f)) satisfies f m^2 <= s < f (m+1)^2, and
+//   EUC_2D  nint(sqrt(s)) = m + [s > m^2 + m]  (sqrt(s) is never m + 1/2)
+//   CEIL_2D               = m + [s > m^2]
+//   ATT                   = m + [s > 10 m^2]
+// The FP64 square root starts m within one of the floor; the integer
+// comparisons make it exact (s < 2^45 here).
+__host__ __device__ inline int32_t metric_dist(long long s, uint32_t metric)
+{
+    const long long f = metric == L2O_ATT ? 10 : 1;
+    const long long e = metric == L2O_EUC_2D ? 1 : 0;
+    long long m = (long long)sqrt((double)s * (metric == L2O_ATT ? 0.1 : 1.0));
+    if (f * m * m > s) m -= 1;
+    else if (f * (m + 1) * (m + 1) <= s) m += 1;
+    return (int32_t)(m + (s > f * m * m + e * m ? 1 : 0));
+}
+
+// Tier-1 factor g of the sweep's integer bound.  |dx| + |dy| <= sqrt(2) d for
+// the Euclidean length d, and d < D + 1/2 (EUC_2D), d <= D (CEIL_2D),
+// d <= sqrt(10) D (ATT), so a candidate with gain <= cap has
+//   L1(a,b) + L1(a+1,b+1) <= sqrt(g) (cap + 1 + D[a+1] + D[b+1])
+// with g = 2, 2, 20.
+__host__ __device__ inline int32_t tier1_g(uint32_t metric) { return metric == L2O_ATT ? 20 : 2; }
+
+// ceil(sqrt(g) * d) for 0 <= d < 2^24, g in {2, 20}.  Neither is a square, so
+// for d >= 1 sqrt(g) d is irrational and the ceiling is isqrt(g d^2) + 1;
+// isqrt is exact after the correction.
+__host__ __device__ inline int32_t ceil_sqrtg(int32_t d, int32_t g)
 {
     if (d <= 0) return 0;
-    const long long s = 2ll * d * d;
+    const long long s = (long long)g * d * d;
     long long k = (long long)sqrt((double)s);
     if (k * k > s) k -= 1;
     else if ((k + 1) * (k + 1) <= s) k += 1;
     return (int32_t)(k + 1);
 }
+
+__host__ __device__ inline int32_t ceil_sqrt2(int32_t d) { return ceil_sqrtg(d, 2); }
 
 // Sweep tile shape: each workgroup has `waves` wave64s, each lane owns `k`
 // adjacent candidate columns for `t` rows.  A workgroup covers 64*waves*k

@@ -4,8 +4,10 @@
 // Per instance (n cities) HBM holds, 16-byte aligned:
 //   pts[2][n]  l2o_pt {x, y, D, W}   tour-ordered points (ping-pong): integer
 //                                     coordinates shifted into [0, 2^22),
-//                                     D[k] = exact nint(d(P[k-1], P[k])), D[0]
-//                                     the closing edge, W[k] = ceil(sqrt(2) D[k])
+//                                     D[k] = exact metric length of the edge
+//                                     P[k-1] -> P[k] (EUC_2D, CEIL_2D or ATT),
+//                                     D[0] the closing edge,
+//                                     W[k] = ceil(sqrt(g) D[k]), g = 2 or 20
 //   tour[2][n] uint32                 city ids in the same order (ping-pong)
 //   bpts[n], btour[n]                 best tour of an ILS run
 //   log[log_cap]                      applied moves {i, J, gain}
@@ -32,8 +34,8 @@ enum l2o_phase : uint32_t {
 
 struct __attribute__((aligned(16))) l2o_pt {
     int32_t x, y;  // coordinates, 0 <= x, y < 2^22
-    int32_t d;     // exact length of the edge into this point
-    int32_t w;     // ceil(sqrt(2) * d), the edge's tier-1 sweep bound
+    int32_t d;     // exact metric length of the edge into this point
+    int32_t w;     // ceil(sqrt(g) * d), the edge's tier-1 sweep bound
 };
 
 struct l2o_move {
@@ -56,6 +58,9 @@ struct l2o_inst {
     uint32_t ils_rounds;  // 0 = local search only
     uint32_t kick;
     uint32_t have_best;
+    uint32_t metric;      // edge length: L2O_EUC_2D, L2O_CEIL_2D or L2O_ATT
+    float est_scale;      // FP32 tier: D ~ sqrt(s) * est_scale + est_add
+    float est_add;
     // ---- dynamic
     uint32_t phase;
     uint32_t parity;

@@ -15,6 +15,13 @@ LIB_PATH = os.path.join(HERE, "liblogo2opt.so")
 KICK_DOUBLE_BRIDGE = 0
 KICK_REFERENCE = 1
 MAX_N = 1 << 20
+METRIC_EUC_2D = 0
+METRIC_CEIL_2D = 1
+METRIC_ATT = 2
+METRICS = {"EUC_2D": METRIC_EUC_2D, "CEIL_2D": METRIC_CEIL_2D, "ATT": METRIC_ATT}
+TSP_ROUND = 1
+EFORMAT = -9
+EIO = -10
 
 # every entry point the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -25,7 +32,8 @@ EXPORTS = [
     "logo2opt_engine_poll_done", "logo2opt_engine_stats", "logo2opt_engine_set_timing", "logo2opt_engine_set_graphs", "logo2opt_gen_uniform", "logo2opt_gen_clustered",
     "logo2opt_random_perm", "logo2opt_nn_tour", "logo2opt_xy_to_coords", "logo2opt_sqrt_ulp_check",
     "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
-    "logo2opt_engine_time_sweep",
+    "logo2opt_engine_time_sweep", "logo2opt_create_metric", "logo2opt_tsp_read", "logo2opt_tour_read",
+    "logo2opt_tour_write", "logo2opt_host_route_length",
 ]
 
 
@@ -50,7 +58,12 @@ class Stats(C.Structure):
 
 class Job(C.Structure):
     _fields_ = [("ils_rounds", C.c_uint32), ("kick", C.c_uint32), ("seed", C.c_uint64),
-                ("log_cap", C.c_uint32), ("reserved", C.c_uint32)]
+                ("log_cap", C.c_uint32), ("metric", C.c_uint32)]
+
+
+class TspInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("n", C.c_uint32), ("metric", C.c_int32), ("offset_x", C.c_double),
+                ("offset_y", C.c_double), ("integral", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -82,6 +95,11 @@ def load():
     L.logo2opt_strerror.argtypes = [C.c_int]
     L.logo2opt_strerror.restype = C.c_char_p
     L.logo2opt_create.argtypes = [C.c_int, C.c_uint32, vp, C.POINTER(vp)]
+    L.logo2opt_create_metric.argtypes = [C.c_int, C.c_uint32, vp, C.c_int, C.POINTER(vp)]
+    L.logo2opt_tsp_read.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(TspInfo), vp, C.c_uint32]
+    L.logo2opt_tour_read.argtypes = [C.c_char_p, C.c_uint32, u32p]
+    L.logo2opt_tour_write.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, u32p, C.c_uint64, C.c_int]
+    L.logo2opt_host_route_length.argtypes = [C.c_uint32, vp, C.c_int, u32p, C.POINTER(C.c_uint64)]
     L.logo2opt_destroy.argtypes = [vp]
     L.logo2opt_step.argtypes = [vp, u32p, C.POINTER(Best2Out), C.POINTER(ProcessTime)]
     L.logo2opt_optimize.argtypes = [vp, u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp, C.c_uint64,
@@ -185,6 +203,42 @@ def sqrt_ulp_check(lo_bits, hi_bits, device=0):
     return m.value
 
 
+# ---- TSPLIB files ------------------------------------------------------------
+def tsp_read(path, round_coords=False):
+    """Reads a TSPLIB .tsp file: returns (info dict, city_coords array).  The
+    coordinates are translated so that the minimum is 0 (info offset_x/_y)."""
+    L = load()
+    info = TspInfo()
+    flags = TSP_ROUND if round_coords else 0
+    rc = L.logo2opt_tsp_read(os.fsencode(path), flags, C.byref(info), None, 0)
+    _check(rc, f"tsp_read({path})")
+    c = (CityCoords * info.n)()
+    _check(L.logo2opt_tsp_read(os.fsencode(path), flags, C.byref(info), C.cast(c, vp), info.n), f"tsp_read({path})")
+    d = dict(name=info.name.decode(errors="replace"), n=info.n, metric=info.metric, offset_x=info.offset_x,
+             offset_y=info.offset_y, integral=bool(info.integral))
+    return d, c
+
+
+def tour_read(path, n):
+    r = np.zeros(n, np.uint32)
+    _check(load().logo2opt_tour_read(os.fsencode(path), n, r), f"tour_read({path})")
+    return r
+
+
+def tour_write(path, route, name="tour", length=0, headerless=False):
+    route = np.ascontiguousarray(route, np.uint32)
+    _check(load().logo2opt_tour_write(os.fsencode(path), name.encode(), len(route), route, length,
+                                      1 if headerless else 0), f"tour_write({path})")
+
+
+def host_route_length(coords, route, metric=METRIC_EUC_2D):
+    route = np.ascontiguousarray(route, np.uint32)
+    L = C.c_uint64()
+    _check(load().logo2opt_host_route_length(len(route), C.cast(coords, vp), metric, route, C.byref(L)),
+           "host_route_length")
+    return int(L.value)
+
+
 # ---- sweep tile shapes -------------------------------------------------------
 def tuning_shapes():
     out = np.zeros(3 * 64, np.int32)
@@ -218,14 +272,18 @@ def tuning_load(path, table=TABLE_SINGLE):
 
 # ---- single-instance context (the CUDASolver replacement) -----------------
 class Context:
-    def __init__(self, coords, n=None, device=0):
+    def __init__(self, coords, n=None, device=0, metric=None):
         L = load()
         if n is None:
             n = len(coords)
         self.n = n
         self._coords = coords  # keep alive
         h = vp()
-        _check(L.logo2opt_create(device, n, C.cast(coords, vp), C.byref(h)), "logo2opt_create")
+        if metric is None:
+            _check(L.logo2opt_create(device, n, C.cast(coords, vp), C.byref(h)), "logo2opt_create")
+        else:
+            _check(L.logo2opt_create_metric(device, n, C.cast(coords, vp), metric, C.byref(h)),
+                   "logo2opt_create_metric")
         self.h = h
 
     def close(self):
@@ -326,10 +384,10 @@ class Engine:
         j = Job(ils_rounds, 0, 0, log_cap, 0)
         return int(load().logo2opt_engine_bytes_for(n, C.byref(j)))
 
-    def add(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, log_cap=0):
+    def add(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, log_cap=0, metric=METRIC_EUC_2D):
         """Returns the slot id, or None when the engine is full (EAGAIN)."""
         route = np.ascontiguousarray(route, np.uint32)
-        j = Job(ils_rounds, kick, seed, log_cap, 0)
+        j = Job(ils_rounds, kick, seed, log_cap, metric)
         s = C.c_int32()
         rc = load().logo2opt_engine_add(self.h, len(route), C.cast(coords, vp), route, C.byref(j), C.byref(s))
         if rc == -6:
@@ -435,9 +493,9 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0):
+    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D):
         route = np.ascontiguousarray(route, np.uint32)
-        j = Job(ils_rounds, kick, seed, 0, 0)
+        j = Job(ils_rounds, kick, seed, 0, metric)
         rid = C.c_int64()
         rc = load().logo2opt_server_submit(self.h, len(route), C.cast(coords, vp), route, C.byref(j), C.byref(rid))
         if rc == -6:

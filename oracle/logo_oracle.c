@@ -171,10 +171,34 @@ uint32_t orc_nint_sqrt(uint64_t s)
     return (uint32_t)(k + (s > k * k + k ? 1u : 0u));
 }
 
-static inline uint32_t dist_xy(int64_t x1, int64_t y1, int64_t x2, int64_t y2)
+/* TSPLIB95 CEIL_2D and ATT (Reinelt, "TSPLIB 95", section 2.1 and 2.5) on
+ * integer s = dx^2 + dy^2, written as ceilings of square roots:
+ *   ceil(sqrt(s))      = isqrt(s - 1) + 1 for s >= 1 (the smallest m with
+ *                        m^2 >= s), 0 for s = 0;
+ *   ceil(sqrt(s / 10)) = isqrt(c - 1) + 1 with c = ceil(s / 10) (m^2 >= s/10
+ *                        iff m^2 >= c for integer m), 0 for s = 0.
+ * TSPLIB's ATT recipe t = nint(r), r = sqrt(s/10), d = t + [t < r] equals
+ * ceil(r).  The reference has neither (tsp_loader.c:49-162 reads every file
+ * as EUC_2D). */
+uint32_t orc_metric_len(uint64_t s, int metric)
+{
+    if (metric == ORC_METRIC_CEIL_2D) return s == 0 ? 0u : (uint32_t)(isqrt_u64(s - 1) + 1);
+    if (metric == ORC_METRIC_ATT) {
+        uint64_t c = (s + 9) / 10;
+        return c == 0 ? 0u : (uint32_t)(isqrt_u64(c - 1) + 1);
+    }
+    return orc_nint_sqrt(s);
+}
+
+void orc_metric_len_batch(const uint64_t *s, uint32_t *out, uint64_t count, int metric)
+{
+    for (uint64_t k = 0; k < count; k++) out[k] = orc_metric_len(s[k], metric);
+}
+
+static inline uint32_t dist_xy(int64_t x1, int64_t y1, int64_t x2, int64_t y2, int metric)
 {
     int64_t dx = x1 - x2, dy = y1 - y2;
-    return orc_nint_sqrt((uint64_t)(dx * dx + dy * dy));
+    return orc_metric_len((uint64_t)(dx * dx + dy * dy), metric);
 }
 
 void orc_nint_sqrt_batch(const uint64_t *s, uint32_t *out, uint64_t count)
@@ -182,18 +206,28 @@ void orc_nint_sqrt_batch(const uint64_t *s, uint32_t *out, uint64_t count)
     for (uint64_t k = 0; k < count; k++) out[k] = orc_nint_sqrt(s[k]);
 }
 
+uint32_t orc_dist_m(const int32_t *xy, uint32_t a, uint32_t b, int metric)
+{
+    return dist_xy(xy[2 * a], xy[2 * a + 1], xy[2 * b], xy[2 * b + 1], metric);
+}
+
 uint32_t orc_dist(const int32_t *xy, uint32_t a, uint32_t b)
 {
-    return dist_xy(xy[2 * a], xy[2 * a + 1], xy[2 * b], xy[2 * b + 1]);
+    return orc_dist_m(xy, a, b, ORC_METRIC_EUC_2D);
 }
 
 /* headers.h:170-189: sum of consecutive edges plus the closing edge. */
-uint64_t orc_route_length(uint32_t n, const int32_t *xy, const uint32_t *route)
+uint64_t orc_route_length_m(uint32_t n, const int32_t *xy, const uint32_t *route, int metric)
 {
     uint64_t d = 0;
-    for (uint32_t k = 1; k < n; k++) d += orc_dist(xy, route[k - 1], route[k]);
-    if (n > 0) d += orc_dist(xy, route[n - 1], route[0]);
+    for (uint32_t k = 1; k < n; k++) d += orc_dist_m(xy, route[k - 1], route[k], metric);
+    if (n > 0) d += orc_dist_m(xy, route[n - 1], route[0], metric);
     return d;
+}
+
+uint64_t orc_route_length(uint32_t n, const int32_t *xy, const uint32_t *route)
+{
+    return orc_route_length_m(n, xy, route, ORC_METRIC_EUC_2D);
 }
 
 /* construction.c:152-183 (returns an error code instead of exit(-1)). */
@@ -218,6 +252,7 @@ typedef struct {
     const int64_t *px, *py; /* tour-ordered coordinates, cpuSolver.cpp:67-79 */
     const uint32_t *dk;     /* dk[k] = d(P[k-1],P[k]) for k >= 1            */
     uint32_t i_lo, i_hi;    /* i in [i_lo, i_hi)                            */
+    int metric;
     int32_t best;
     uint32_t bi, bj;
 } sweep_job;
@@ -230,14 +265,15 @@ static void *sweep_range(void *arg)
     const int64_t *px = jb->px, *py = jb->py;
     const uint32_t *dk = jb->dk;
     uint32_t n = jb->n;
+    const int metric = jb->metric;
     int32_t best = 0;
     uint32_t bi = 0, bj = 0;
     for (uint32_t i = jb->i_lo; i < jb->i_hi; i++) {
         int64_t xi = px[i], yi = py[i], xh = px[i - 1], yh = py[i - 1];
         int32_t c1 = (int32_t)dk[i];
         for (uint32_t J = i + 2; J <= n - 1; J++) {
-            int32_t g = (int32_t)dist_xy(xi, yi, px[J], py[J]) +
-                        (int32_t)dist_xy(xh, yh, px[J - 1], py[J - 1]) - c1 -
+            int32_t g = (int32_t)dist_xy(xi, yi, px[J], py[J], metric) +
+                        (int32_t)dist_xy(xh, yh, px[J - 1], py[J - 1], metric) - c1 -
                         (int32_t)dk[J];
             if (g < best) {
                 best = g;
@@ -252,8 +288,8 @@ static void *sweep_range(void *arg)
     return NULL;
 }
 
-int orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
-             orc_best2_out *out)
+int orc_step_m(uint32_t n, const int32_t *xy, const uint32_t *route, int threads, int metric,
+               orc_best2_out *out)
 {
     out->i = 0;
     out->j = 0;
@@ -266,8 +302,8 @@ int orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
         px[k] = xy[2 * route[k]];
         py[k] = xy[2 * route[k] + 1];
     }
-    dk[0] = dist_xy(px[n - 1], py[n - 1], px[0], py[0]);
-    for (uint32_t k = 1; k < n; k++) dk[k] = dist_xy(px[k - 1], py[k - 1], px[k], py[k]);
+    dk[0] = dist_xy(px[n - 1], py[n - 1], px[0], py[0], metric);
+    for (uint32_t k = 1; k < n; k++) dk[k] = dist_xy(px[k - 1], py[k - 1], px[k], py[k], metric);
 
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
@@ -300,6 +336,7 @@ int orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
         jobs[t].dk = dk;
         jobs[t].i_lo = lo;
         jobs[t].i_hi = hi;
+        jobs[t].metric = metric;
         lo = hi;
     }
     if (threads == 1) {
@@ -330,6 +367,12 @@ int orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
     return 0;
 }
 
+int orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
+             orc_best2_out *out)
+{
+    return orc_step_m(n, xy, route, threads, ORC_METRIC_EUC_2D, out);
+}
+
 /* twoOptLocalSolver.cpp:41-48 in closed form:
  *   rotate(begin, begin+i, end); reverse(begin+(J-i), end)
  * gives r' = r[i..J-1] ++ r[i-1], ..., r[0] ++ r[n-1], ..., r[J]. */
@@ -346,18 +389,18 @@ void orc_apply(uint32_t n, uint32_t *route, uint32_t i, uint32_t j)
 }
 
 /* twoOptLocalSolver.cpp:22-64: step, apply while the gain is negative. */
-uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
-                          int threads, uint64_t max_moves,
-                          orc_best2_out *log, uint64_t log_cap,
-                          uint64_t *sweeps, uint64_t *length)
+uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
+                            int threads, int metric, uint64_t max_moves,
+                            orc_best2_out *log, uint64_t log_cap,
+                            uint64_t *sweeps, uint64_t *length)
 {
     uint64_t moves = 0, sw = 0;
-    uint64_t L = orc_route_length(n, xy, route);
+    uint64_t L = orc_route_length_m(n, xy, route, metric);
     for (;;) {
         if (max_moves && moves >= max_moves) break;
         orc_best2_out o;
         sw++;
-        if (!orc_step(n, xy, route, threads, &o)) break;
+        if (!orc_step_m(n, xy, route, threads, metric, &o)) break;
         orc_apply(n, route, o.i, o.j);
         L = (uint64_t)((int64_t)L + o.minchange);
         if (log && moves < log_cap) log[moves] = o;
@@ -366,6 +409,15 @@ uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
     if (sweeps) *sweeps = sw;
     if (length) *length = L;
     return moves;
+}
+
+uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
+                          int threads, uint64_t max_moves,
+                          orc_best2_out *log, uint64_t log_cap,
+                          uint64_t *sweeps, uint64_t *length)
+{
+    return orc_local_search_m(n, xy, route, threads, ORC_METRIC_EUC_2D, max_moves, log, log_cap,
+                              sweeps, length);
 }
 
 /* Build-defined kick: three distinct cut points from 1..n-1, sorted. */
@@ -425,14 +477,14 @@ void orc_ref_kick(uint32_t n, uint32_t *route, orc_rng_t *rng)
 /* ILSGlobalSolverSequential.cpp:60-100 with a fixed round count in place of
  * the target/time stop rule: LO, accept iff strictly shorter else revert to
  * the best, kick the best. */
-uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
-                 uint32_t rounds, int kick_kind, int threads,
-                 uint64_t *round_lengths, uint64_t *total_moves)
+uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                   uint32_t rounds, int kick_kind, int threads, int metric,
+                   uint64_t *round_lengths, uint64_t *total_moves)
 {
     orc_rng_t rng;
     orc_rng_seed(&rng, seed);
     uint64_t L, tm = 0;
-    tm += orc_local_search(n, xy, route, threads, 0, NULL, 0, NULL, &L);
+    tm += orc_local_search_m(n, xy, route, threads, metric, 0, NULL, 0, NULL, &L);
     uint64_t Lbest = L;
     uint32_t *best = (uint32_t *)malloc(sizeof(uint32_t) * n);
     uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * n);
@@ -444,7 +496,7 @@ uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
             orc_ref_kick(n, cur, &rng);
         else
             orc_double_bridge(n, cur, &rng, NULL);
-        tm += orc_local_search(n, xy, cur, threads, 0, NULL, 0, NULL, &L);
+        tm += orc_local_search_m(n, xy, cur, threads, metric, 0, NULL, 0, NULL, &L);
         if (L < Lbest) {
             Lbest = L;
             memcpy(best, cur, sizeof(uint32_t) * n);
@@ -456,4 +508,12 @@ uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
     free(cur);
     if (total_moves) *total_moves = tm;
     return Lbest;
+}
+
+uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                 uint32_t rounds, int kick_kind, int threads,
+                 uint64_t *round_lengths, uint64_t *total_moves)
+{
+    return orc_ils_m(n, xy, route, seed, rounds, kick_kind, threads, ORC_METRIC_EUC_2D,
+                     round_lengths, total_moves);
 }

@@ -21,13 +21,16 @@
  * Deliberate differences from the reference (its arithmetic is not exact):
  *   - every edge length is the exact TSPLIB EUC_2D value nint(sqrt(dx^2+dy^2))
  *     on integer coordinates (the reference mixes float sqrtf, +0.5f and
- *     truncation, headers.h:126-142, cpuSolver.cpp:133-138);
+ *     truncation, headers.h:126-142, cpuSolver.cpp:133-138), or on request
+ *     TSPLIB CEIL_2D / ATT, which the reference does not implement;
  *   - randomness comes from xoshiro256** seeded by SplitMix64 instead of
  *     libc rand() (common.c:253-259), so runs are reproducible;
  *   - the ILS kick used by the configs is a seeded double-bridge.
  *
- * Pins (see tests/test_oracle.py): the nint interval structure over all
- * s < 2^24, Python math.isqrt golden vectors for s < 2^46, the
+ * Pins (see tests/test_oracle.py, tests/test_tsplib.py): the nint interval
+ * structure over all s < 2^24, Python math.isqrt golden vectors for s < 2^46,
+ * the TSPLIB95 double-precision CEIL_2D / ATT recipes, the TSPLIB known
+ * optima of att532 / pla7397 as lower bounds, the
  * mona-lisa100K tour length 5,757,191 stated by the reference's own tour
  * file, lattice closed forms, brute-force enumeration on tiny tours,
  * std::rotate/std::reverse for the apply step, and the reference's own
@@ -65,6 +68,17 @@ uint32_t orc_nint_sqrt(uint64_t s);
 void     orc_nint_sqrt_batch(const uint64_t *s, uint32_t *out, uint64_t count);
 uint32_t orc_dist(const int32_t *xy, uint32_t a, uint32_t b);
 uint64_t orc_route_length(uint32_t n, const int32_t *xy, const uint32_t *route);
+
+/* TSPLIB95 metrics: EUC_2D nint(sqrt(s)), CEIL_2D ceil(sqrt(s)), ATT
+ * ceil(sqrt(s/10)).  The *_m functions take the metric; the unsuffixed ones
+ * are EUC_2D, the reference's only metric. */
+#define ORC_METRIC_EUC_2D  0
+#define ORC_METRIC_CEIL_2D 1
+#define ORC_METRIC_ATT     2
+uint32_t orc_metric_len(uint64_t s, int metric);
+void     orc_metric_len_batch(const uint64_t *s, uint32_t *out, uint64_t count, int metric);
+uint32_t orc_dist_m(const int32_t *xy, uint32_t a, uint32_t b, int metric);
+uint64_t orc_route_length_m(uint32_t n, const int32_t *xy, const uint32_t *route, int metric);
 /* 0 if route is a permutation of 0..n-1 with n >= 3, else -1 */
 int      orc_check_tour(uint32_t n, const uint32_t *route);
 
@@ -79,6 +93,8 @@ typedef struct { uint32_t i; uint32_t j; int32_t minchange; } orc_best2_out;
  * key, so the result does not depend on the thread count. */
 int  orc_step(uint32_t n, const int32_t *xy, const uint32_t *route, int threads,
               orc_best2_out *out);
+int  orc_step_m(uint32_t n, const int32_t *xy, const uint32_t *route, int threads, int metric,
+                orc_best2_out *out);
 /* r' = r[i..J-1] ++ r[i-1..0] ++ r[n-1..J]  (i < J); twoOptLocalSolver.cpp:41-48 */
 void orc_apply(uint32_t n, uint32_t *route, uint32_t i, uint32_t j);
 
@@ -89,6 +105,10 @@ uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
                           int threads, uint64_t max_moves,
                           orc_best2_out *log, uint64_t log_cap,
                           uint64_t *sweeps, uint64_t *length);
+uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
+                            int threads, int metric, uint64_t max_moves,
+                            orc_best2_out *log, uint64_t log_cap,
+                            uint64_t *sweeps, uint64_t *length);
 
 /* Double-bridge A B C D -> A C B D with 1 <= p1 < p2 < p3 <= n-1 drawn from
  * the generator.  cuts (optional) receives p1,p2,p3. */
@@ -110,6 +130,9 @@ void orc_ref_kick(uint32_t n, uint32_t *route, orc_rng_t *rng);
 uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
                  uint32_t rounds, int kick_kind, int threads,
                  uint64_t *round_lengths, uint64_t *total_moves);
+uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                   uint32_t rounds, int kick_kind, int threads, int metric,
+                   uint64_t *round_lengths, uint64_t *total_moves);
 
 #ifdef __cplusplus
 }

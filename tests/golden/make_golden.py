@@ -2,7 +2,7 @@
 
 Run once in a container that has the reference checkout (read as data only):
 
-    python tests/golden/make_golden.py [/root/reference]
+    python tests/golden/make_golden.py [/root/reference] [all|npz|tsplib]
 
 Outputs
   tsplib_int.npz   integral-coordinate EUC_2D TSPLIB instances (TSPLIB/*.tsp)
@@ -14,10 +14,21 @@ Outputs
   nint_golden.npz  random s < 2^46 with nint(sqrt(s)) = (isqrt(4s)+1)//2 from
                    Python's math.isqrt (stdlib; a different formula from the
                    oracle's).
+  tsplib/*.tsp.xz  TSPLIB data files, byte for byte (xz-compressed), for the
+                   .tsp reader tests: berlin52 (EUC_2D), att532 (ATT),
+                   pla7397 and pla33810 (CEIL_2D), pcb3038 (negative values,
+                   exponent notation), vm1084 (non-integral values) and
+                   rat195 (an HTML page in place of the data).
+  monalisa_tour.json  the header lines, line ending, size and SHA-256 of
+                   TSPLIB/monalisa_5757191.tour, so the tests can rebuild that
+                   file byte for byte from monalisa.npz's tour.
 
 Nothing here imports or runs reference code; the .tsp/.tour files are parsed
 as text.
 """
+import hashlib
+import json
+import lzma
 import math
 import os
 import re
@@ -69,9 +80,36 @@ def read_tour(path):
     return name, np.array(ids, np.uint32)
 
 
+TSPLIB_FILES = ["berlin52", "att532", "pla7397", "pla33810", "pcb3038", "vm1084", "rat195"]
+
+
+def write_tsplib(tsplib):
+    out = os.path.join(HERE, "tsplib")
+    os.makedirs(out, exist_ok=True)
+    for name in TSPLIB_FILES:
+        with open(os.path.join(tsplib, name + ".tsp"), "rb") as f:
+            data = f.read()
+        with open(os.path.join(out, name + ".tsp.xz"), "wb") as f:
+            f.write(lzma.compress(data, preset=9))
+    with open(os.path.join(tsplib, "monalisa_5757191.tour"), "rb") as f:
+        raw = f.read()
+    eol = "\r\n" if b"\r\n" in raw else "\n"
+    text = raw.decode()
+    head = text[:text.index("TOUR_SECTION") + len("TOUR_SECTION") + len(eol)]
+    meta = dict(header=head, eol=eol, bytes=len(raw), sha256=hashlib.sha256(raw).hexdigest())
+    with open(os.path.join(HERE, "monalisa_tour.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    what = sys.argv[2] if len(sys.argv) > 2 else "all"
     tsplib = os.path.join(ref, "TSPLIB")
+    if what in ("all", "tsplib"):
+        write_tsplib(tsplib)
+    if what not in ("all", "npz"):
+        print("wrote fixtures to", HERE)
+        return
 
     arrays = {}
     names = []

@@ -54,6 +54,19 @@ def _load_oracle():
     L.orc_ils.argtypes = [C.c_uint32, i32p, u32p, C.c_uint64, C.c_uint32, C.c_int, C.c_int,
                           C.c_void_p, C.POINTER(C.c_uint64)]
     L.orc_ils.restype = C.c_uint64
+    L.orc_metric_len.argtypes = [C.c_uint64, C.c_int]
+    L.orc_metric_len.restype = C.c_uint32
+    L.orc_metric_len_batch.argtypes = [u64p, u32p, C.c_uint64, C.c_int]
+    L.orc_route_length_m.argtypes = [C.c_uint32, i32p, u32p, C.c_int]
+    L.orc_route_length_m.restype = C.c_uint64
+    L.orc_step_m.argtypes = [C.c_uint32, i32p, u32p, C.c_int, C.c_int, C.POINTER(Best2Out)]
+    L.orc_local_search_m.argtypes = [C.c_uint32, i32p, u32p, C.c_int, C.c_int, C.c_uint64,
+                                     C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint64)]
+    L.orc_local_search_m.restype = C.c_uint64
+    L.orc_ils_m.argtypes = [C.c_uint32, i32p, u32p, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                            C.c_void_p, C.POINTER(C.c_uint64)]
+    L.orc_ils_m.restype = C.c_uint64
     return L
 
 
@@ -96,13 +109,23 @@ def nint_sqrt(s):
     return out
 
 
-def route_length(xy, route):
-    return int(lib.orc_route_length(len(route), xy, np.ascontiguousarray(route, np.uint32)))
+METRIC_EUC_2D, METRIC_CEIL_2D, METRIC_ATT = 0, 1, 2
 
 
-def step(xy, route, threads=1):
+def metric_len(s, metric):
+    s = np.ascontiguousarray(s, np.uint64)
+    out = np.zeros(len(s), np.uint32)
+    lib.orc_metric_len_batch(s, out, len(s), metric)
+    return out
+
+
+def route_length(xy, route, metric=METRIC_EUC_2D):
+    return int(lib.orc_route_length_m(len(route), xy, np.ascontiguousarray(route, np.uint32), metric))
+
+
+def step(xy, route, threads=1, metric=METRIC_EUC_2D):
     o = Best2Out()
-    ok = lib.orc_step(len(route), xy, np.ascontiguousarray(route, np.uint32), threads, C.byref(o))
+    ok = lib.orc_step_m(len(route), xy, np.ascontiguousarray(route, np.uint32), threads, metric, C.byref(o))
     return bool(ok), (o.i, o.j, o.minchange)
 
 
@@ -112,23 +135,23 @@ def apply(route, i, j):
     return r
 
 
-def local_search(xy, route, threads=1, max_moves=0, log_cap=1 << 20):
+def local_search(xy, route, threads=1, max_moves=0, log_cap=1 << 20, metric=METRIC_EUC_2D):
     r = np.array(route, np.uint32)
     log = np.zeros(3 * log_cap, np.int32)
     sweeps = C.c_uint64()
     length = C.c_uint64()
-    moves = lib.orc_local_search(len(r), xy, r, threads, max_moves, log.ctypes.data, log_cap,
-                                 C.byref(sweeps), C.byref(length))
+    moves = lib.orc_local_search_m(len(r), xy, r, threads, metric, max_moves, log.ctypes.data, log_cap,
+                                   C.byref(sweeps), C.byref(length))
     m = min(moves, log_cap)
     return r, int(length.value), int(moves), int(sweeps.value), log[:3 * m].reshape(m, 3).copy()
 
 
-def ils(xy, route, seed, rounds, kick="double_bridge", threads=1):
+def ils(xy, route, seed, rounds, kick="double_bridge", threads=1, metric=METRIC_EUC_2D):
     r = np.array(route, np.uint32)
     rl = np.zeros(rounds + 1, np.uint64)
     tm = C.c_uint64()
-    best = lib.orc_ils(len(r), xy, r, seed, rounds, 0 if kick == "double_bridge" else 1, threads,
-                       rl.ctypes.data, C.byref(tm))
+    best = lib.orc_ils_m(len(r), xy, r, seed, rounds, 0 if kick == "double_bridge" else 1, threads, metric,
+                         rl.ctypes.data, C.byref(tm))
     return r, int(best), rl, int(tm.value)
 
 

@@ -115,3 +115,41 @@ def test_header_is_c_and_cpp_adapter_compiles(tmp_path):
     r = subprocess.run([str(exe2)], capture_output=True, text=True)
     # n = 2 is rejected before any device work: the adapter throws, never exits
     assert r.returncode == 0 and "invalid argument" in r.stdout
+
+
+def test_cpp_adapter_file_helpers(tmp_path):
+    # readFileCoords / saveRouteToAFile / routeInitFromFile under the
+    # reference's names (host only, no device work)
+    pkg = os.path.dirname(L.LIB_PATH)
+    inc = os.path.join(ROOT, "include")
+    tsp = tmp_path / "sq.tsp"
+    tsp.write_text("NAME : sq\nTYPE : TSP\nDIMENSION : 4\nEDGE_WEIGHT_TYPE : ATT\nNODE_COORD_SECTION\n"
+                   "1 10 10\n2 40 10\n3 40 50\n4 10 50\nEOF\n")
+    src = tmp_path / "f.cpp"
+    src.write_text(
+        '#include "hip_local_solver.h"\n#include <cstdio>\n'
+        'int main(int, char **argv) {\n'
+        '  int metric = -1;\n'
+        '  std::vector<logo2opt_city_coords> c = logo2opt::readFileCoords(argv[1], &metric);\n'
+        '  std::vector<uint32_t> r = {2, 0, 3, 1};\n'
+        '  uint64_t len = 0;\n'
+        '  if (logo2opt_host_route_length(4, c.data(), metric, r.data(), &len)) return 2;\n'
+        '  logo2opt::saveRouteToAFile(r, argv[2], (unsigned long)len);\n'
+        '  std::vector<uint32_t> back;\n'
+        '  logo2opt::routeInitFromFile(back, 4, (std::string(argv[2]) + "_" + std::to_string(len) + ".tour").c_str());\n'
+        '  std::printf("%zu %d %g %g %llu %d\\n", c.size(), metric, c[2].x, c[2].y, (unsigned long long)len,\n'
+        '              (int)(back == r));\n'
+        '  try { logo2opt::readFileCoords(argv[3]); } catch (const std::runtime_error &e) { std::puts(e.what()); }\n'
+        '  return 0; }\n')
+    exe = tmp_path / "f"
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-I", inc, "-I", os.path.join(pkg, "csrc"),
+                    str(src), "-o", str(exe), "-L", pkg, "-llogo2opt", f"-Wl,-rpath,{pkg}"], check=True)
+    out = subprocess.run([str(exe), str(tsp), str(tmp_path / "best"), str(tmp_path / "none.tsp")],
+                         capture_output=True, text=True, check=True).stdout.split("\n")
+    # tour 3-1-4-2 has Euclidean edges 50, 40, 50, 40; ATT is the smallest m
+    # with 10 m^2 >= d^2: 16, 13, 16, 13
+    att = sum(next(m for m in range(100) if 10 * m * m >= d * d) for d in (50, 40, 50, 40))
+    assert att == 58
+    assert out[0].split() == ["4", "2", "30", "40", str(att), "1"]
+    assert (tmp_path / f"best_{att}.tour").read_text().split() == ["3", "1", "4", "2"]
+    assert "could not be opened" in out[1]

@@ -1,0 +1,297 @@
+"""TSPLIB metrics and files (CPU).
+
+Pins the oracle's CEIL_2D / ATT lengths against the TSPLIB95 definitions
+evaluated in IEEE double (Reinelt, "TSPLIB 95": CEIL_2D = ceil of the
+Euclidean distance; ATT: r = sqrt((dx^2+dy^2)/10), t = nint(r), d = t+1 if
+t < r else t), exhaustively for s < 2^24 and on random s < 2^45; the library's
+host lengths (the same function the kernels use) against the oracle; the .tsp
+reader on TSPLIB files the reference ships (tests/golden/tsplib, byte-for-byte
+copies) with the known optima the reference keys by file name
+(src/utils/cut.c:255-303) as lower bounds; and the .tour reader/writer against
+the reference's mona-lisa tour (src/utils/common.c:109-192 read/write the
+header-less form).
+"""
+import hashlib
+import json
+import lzma
+import os
+
+import numpy as np
+import pytest
+
+import logo2opt as L2
+import oracle_lib as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+KNOWN_OPT = {"att532": 27686, "pla7397": 23260728, "pla33810": 66048945, "berlin52": 7542, "pcb3038": 137694}
+
+
+def fixture_tsp(name, tmp_path):
+    with open(os.path.join(GOLD, "tsplib", name + ".tsp.xz"), "rb") as f:
+        data = lzma.decompress(f.read())
+    p = tmp_path / (name + ".tsp")
+    p.write_bytes(data)
+    return str(p)
+
+
+def xy_of_coords(c):
+    a = np.frombuffer(c, dtype=np.float32).reshape(-1, 2)
+    return np.ascontiguousarray(a.astype(np.int32).reshape(-1))
+
+
+# ---- metric definitions ------------------------------------------------------
+def tsplib_ceil2d(s):
+    return np.ceil(np.sqrt(s.astype(np.float64))).astype(np.int64)
+
+
+def tsplib_att(s):
+    r = np.sqrt(s.astype(np.float64) / 10.0)
+    t = np.floor(r + 0.5)
+    return np.where(t < r, t + 1, t).astype(np.int64)
+
+
+def test_ceil2d_and_att_exhaustive_below_2_24():
+    s = np.arange(0, 1 << 24, dtype=np.uint64)
+    assert np.array_equal(orc.metric_len(s, orc.METRIC_CEIL_2D), tsplib_ceil2d(s))
+    assert np.array_equal(orc.metric_len(s, orc.METRIC_ATT), tsplib_att(s))
+
+
+def test_metrics_random_large_and_breakpoints():
+    rng = np.random.default_rng(7)
+    s = rng.integers(0, 1 << 45, 400000, dtype=np.int64)
+    k = rng.integers(1, 1 << 22, 100000, dtype=np.int64)
+    # squares, 10 m^2 and their neighbours: where the ceilings step
+    s = np.concatenate([s, k * k - 1, k * k, k * k + 1, 10 * k * k - 1, 10 * k * k, 10 * k * k + 1]).astype(np.uint64)
+    assert np.array_equal(orc.metric_len(s, orc.METRIC_CEIL_2D), tsplib_ceil2d(s))
+    assert np.array_equal(orc.metric_len(s, orc.METRIC_ATT), tsplib_att(s))
+    # EUC_2D through the same entry point is the pinned nint
+    assert np.array_equal(orc.metric_len(s, orc.METRIC_EUC_2D), orc.nint_sqrt(s))
+
+
+def test_metric_orderings():
+    # ceil >= nint >= floor and ATT = ceil(sqrt(s)/sqrt(10)) is about a third
+    s = np.random.default_rng(3).integers(1, 1 << 40, 10000).astype(np.uint64)
+    e = orc.metric_len(s, orc.METRIC_EUC_2D).astype(np.int64)
+    c = orc.metric_len(s, orc.METRIC_CEIL_2D).astype(np.int64)
+    a = orc.metric_len(s, orc.METRIC_ATT).astype(np.int64)
+    assert np.all(c >= e) and np.all(c - e <= 1)
+    assert np.all(np.abs(a * np.sqrt(10.0) - np.sqrt(s.astype(np.float64))) <= np.sqrt(10.0) + 1e-6)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_host_route_length_matches_oracle(metric):
+    for seed, n, side in [(1, 50, 1000), (2, 777, 1 << 21), (3, 3000, 10 ** 6)]:
+        xy = orc.gen_uniform(n, side, seed)
+        r = orc.random_perm(n, seed + 10)
+        assert L2.host_route_length(L2.coords_of(xy), r, metric) == orc.route_length(xy, r, metric)
+
+
+def test_oracle_local_search_metrics_invariants():
+    # L' = L + gain for every move, no improving move left, under each metric
+    xy = orc.gen_uniform(120, 5000, 11)
+    r0 = orc.random_perm(120, 12)
+    for metric in (1, 2):
+        r, length, moves, sweeps, log = orc.local_search(xy, r0, metric=metric)
+        assert sweeps == moves + 1
+        assert length == orc.route_length(xy, r, metric)
+        assert length == orc.route_length(xy, r0, metric) + int(log[:, 2].sum())
+        assert not orc.step(xy, r, metric=metric)[0]
+        # brute force: no (i, J) with a negative exact gain
+        P = xy.reshape(-1, 2)[r]
+        n = len(r)
+        for i in range(1, n - 2):
+            for J in range(i + 2, n):
+                def d(a, b):
+                    s = np.uint64((int(P[a][0]) - int(P[b][0])) ** 2 + (int(P[a][1]) - int(P[b][1])) ** 2)
+                    return int(orc.metric_len([s], metric)[0])
+                if i % 13 == 0:  # a sample of rows keeps this quick
+                    assert d(i, J) + d(i - 1, J - 1) - d(i - 1, i) - d(J - 1, J) >= 0
+
+
+# ---- .tsp reader ---------------------------------------------------------------
+def test_reader_berlin52_and_known_values(tmp_path):
+    info, c = L2.tsp_read(fixture_tsp("berlin52", tmp_path))
+    assert (info["name"], info["n"], info["metric"], info["integral"]) == ("berlin52", 52, L2.METRIC_EUC_2D, True)
+    g = np.load(os.path.join(GOLD, "tsplib_int.npz"))
+    xy = xy_of_coords(c).reshape(-1, 2)
+    assert np.array_equal(xy[:, 0] + int(info["offset_x"]), g["berlin52_x"])
+    assert np.array_equal(xy[:, 1] + int(info["offset_y"]), g["berlin52_y"])
+
+
+def test_reader_negative_and_exponent_values(tmp_path):
+    # pcb3038 writes 2.83000e+03 and holds x = -68: translated by the minimum
+    info, c = L2.tsp_read(fixture_tsp("pcb3038", tmp_path))
+    assert info["n"] == 3038 and info["offset_x"] == -68.0 and info["offset_y"] == -5.0
+    g = np.load(os.path.join(GOLD, "tsplib_int.npz"))
+    xy = xy_of_coords(c).reshape(-1, 2)
+    assert xy.min() == 0
+    assert np.array_equal(xy[:, 0] - 68, g["pcb3038_x"]) and np.array_equal(xy[:, 1] - 5, g["pcb3038_y"])
+    # lengths do not depend on the translation
+    r = orc.random_perm(3038, 5)
+    assert L2.host_route_length(c, r) == orc.route_length(orc.xy_of(g["pcb3038_x"], g["pcb3038_y"]), r)
+
+
+def test_reader_att_and_ceil_types_with_optimum_bounds(tmp_path):
+    info, c = L2.tsp_read(fixture_tsp("att532", tmp_path))
+    assert (info["n"], info["metric"]) == (532, L2.METRIC_ATT)
+    xy = xy_of_coords(c)
+    r, length, *_ = orc.local_search(xy, orc.nn_tour(xy), metric=orc.METRIC_ATT, threads=4)
+    # a 2-opt local optimum lies within 15% of the optimum 27686; read as
+    # EUC_2D (what the reference does) the same tour is about sqrt(10) longer
+    assert KNOWN_OPT["att532"] <= length <= 1.15 * KNOWN_OPT["att532"]
+    assert L2.host_route_length(c, r, L2.METRIC_ATT) == length
+    assert L2.host_route_length(c, r, L2.METRIC_EUC_2D) > 3 * KNOWN_OPT["att532"]
+    for name in ("pla7397", "pla33810"):
+        info, c = L2.tsp_read(fixture_tsp(name, tmp_path))
+        assert info["metric"] == L2.METRIC_CEIL_2D and info["integral"]
+        xy = xy_of_coords(c)
+        nn = orc.nn_tour(xy) if info["n"] < 10000 else orc.random_perm(info["n"], 1)
+        Lc = L2.host_route_length(c, nn, L2.METRIC_CEIL_2D)
+        assert Lc >= KNOWN_OPT[name]
+        assert Lc == orc.route_length(xy, nn, orc.METRIC_CEIL_2D)
+        # ceil >= nint edge by edge, by at most one
+        Le = orc.route_length(xy, nn, orc.METRIC_EUC_2D)
+        assert Le <= Lc <= Le + info["n"]
+    assert info["n"] == 33810
+
+
+def test_reader_non_integral_and_html(tmp_path):
+    p = fixture_tsp("vm1084", tmp_path)
+    with pytest.raises(L2.Logo2OptError) as e:
+        L2.tsp_read(p)
+    assert e.value.code == -2  # ECOORD
+    info, c = L2.tsp_read(p, round_coords=True)
+    assert info["n"] == 1084 and not info["integral"]
+    xy = xy_of_coords(c).reshape(-1, 2)
+    # the raw text rounded halves-away and translated gives the same values
+    raw = [l.split() for l in open(p).read().split("\n")]
+    start = next(k for k, t in enumerate(raw) if t and t[0] == "NODE_COORD_SECTION")
+    vals = np.array([[float(t[1]), float(t[2])] for t in raw[start + 1:start + 1 + 1084]])
+    rnd = np.sign(vals) * np.floor(np.abs(vals) + 0.5)
+    assert np.array_equal(xy, (rnd - rnd.min(0)).astype(np.int32))
+    with pytest.raises(L2.Logo2OptError) as e:
+        L2.tsp_read(fixture_tsp("rat195", tmp_path))  # an HTML page, not TSPLIB
+    assert e.value.code == L2.EFORMAT
+
+
+GOOD = """NAME: tiny
+COMMENT : ok
+TYPE : TSP
+DIMENSION : 5
+EDGE_WEIGHT_TYPE : CEIL_2D
+NODE_COORD_SECTION
+3 2 0
+1 0 0
+2 1.0e0 0
+5 0 1
+4 2 1
+EOF
+"""
+
+
+@pytest.mark.parametrize("text,code", [
+    (GOOD, 0),
+    (GOOD.replace("\n", "\r\n"), 0),
+    (GOOD.replace("EOF\n", ""), 0),
+    (GOOD.replace("EOF\n", "DISPLAY_DATA_SECTION\n1 0 0\n2 1 1\nEOF\n"), 0),
+    # keywords in another order, no spaces around ':'
+    ("EDGE_WEIGHT_TYPE:CEIL_2D\nDIMENSION:5\nNAME:tiny\n" + GOOD.split("EDGE_WEIGHT_TYPE : CEIL_2D\n")[1], 0),
+    (GOOD.replace("DIMENSION : 5\n", ""), -9),
+    (GOOD.replace("CEIL_2D", "GEO"), -9),
+    (GOOD.replace("TYPE : TSP", "TYPE : ATSP"), -9),
+    (GOOD.replace("4 2 1", "3 2 1"), -9),       # duplicate id
+    (GOOD.replace("4 2 1", "6 2 1"), -9),       # id out of range
+    (GOOD.replace("4 2 1\n", ""), -9),          # too few entries
+    (GOOD.replace("4 2 1", "4 2 x"), -9),       # not a number
+    (GOOD.replace("4 2 1", "4 2 1 7"), -9),     # extra token
+    (GOOD.replace("4 2 1", "4 2 inf"), -9),
+    (GOOD.replace("EDGE_WEIGHT_TYPE", "NODE_COORD_TYPE : THREED_COORDS\nEDGE_WEIGHT_TYPE"), -9),
+    (GOOD.replace("DIMENSION : 5", "DIMENSION : 2"), -9),
+    ("<html>\n" + GOOD, -9),
+    (GOOD.replace("4 2 1", "4 2 1.5"), -2),     # non-integral
+    (GOOD.replace("4 2 1", "4 4194304 1"), -2),  # span 2^22
+])
+def test_reader_cases(tmp_path, text, code):
+    p = tmp_path / "t.tsp"
+    p.write_bytes(text.encode())
+    if code:
+        with pytest.raises(L2.Logo2OptError) as e:
+            L2.tsp_read(str(p))
+        assert e.value.code == code
+        return
+    info, c = L2.tsp_read(str(p))
+    assert (info["name"], info["n"], info["metric"]) == ("tiny", 5, L2.METRIC_CEIL_2D)
+    assert xy_of_coords(c).tolist() == [0, 0, 1, 0, 2, 0, 2, 1, 0, 1]  # by id
+
+
+def test_reader_missing_file(tmp_path):
+    with pytest.raises(L2.Logo2OptError) as e:
+        L2.tsp_read(str(tmp_path / "nope.tsp"))
+    assert e.value.code == L2.EIO
+
+
+# ---- .tour reader / writer -----------------------------------------------------
+def monalisa_tour_file(tmp_path):
+    meta = json.load(open(os.path.join(GOLD, "monalisa_tour.json")))
+    g = np.load(os.path.join(GOLD, "monalisa.npz"))
+    body = meta["eol"].join(str(int(v) + 1) for v in g["tour"]) + meta["eol"]
+    raw = (meta["header"] + body).encode()
+    # the rebuilt file is the reference's TSPLIB/monalisa_5757191.tour
+    assert len(raw) == meta["bytes"] and hashlib.sha256(raw).hexdigest() == meta["sha256"]
+    p = tmp_path / "monalisa_5757191.tour"
+    p.write_bytes(raw)
+    return str(p), g
+
+
+def test_tour_read_monalisa_kat(tmp_path):
+    p, g = monalisa_tour_file(tmp_path)
+    r = L2.tour_read(p, 100000)
+    assert np.array_equal(r, g["tour"])
+    c = L2.float_coords(g["x"].astype(np.float32), g["y"].astype(np.float32))
+    assert L2.host_route_length(c, r) == int(g["stated_length"][0]) == 5757191
+    with pytest.raises(L2.Logo2OptError) as e:
+        L2.tour_read(p, 99999)  # DIMENSION differs
+    assert e.value.code == L2.EFORMAT
+
+
+def test_tour_write_read_round_trip(tmp_path):
+    r = orc.random_perm(1000, 9)
+    for headerless in (False, True):
+        p = str(tmp_path / f"t{int(headerless)}.tour")
+        L2.tour_write(p, r, name="x", length=123, headerless=headerless)
+        assert np.array_equal(L2.tour_read(p, 1000), r)
+        text = open(p).read().split("\n")
+        if headerless:  # the reference's saveRouteToAFile layout: ids + 1, one per line
+            assert text[:-1] == [str(v + 1) for v in r]
+        else:
+            assert text[:5] == ["NAME : x", "COMMENT : length 123", "TYPE : TOUR", "DIMENSION : 1000",
+                                "TOUR_SECTION"]
+            assert text[-3:] == ["-1", "EOF", ""]
+
+
+@pytest.mark.parametrize("text,code", [
+    ("TYPE : TOUR\nTOUR_SECTION\n1\n2\n3\n4\n-1\n2\n1\n4\n3\n-1\nEOF\n", 0),  # first tour only
+    ("1 2 3 4\n", 0),
+    ("TYPE : TSP\nTOUR_SECTION\n1\n2\n3\n4\n-1\n", -9),
+    ("1\n2\n3\n", -9),
+    ("1\n2\n3\n4\n5\n", -9),
+    ("1\n2\n2\n4\n", -3),
+    ("0\n1\n2\n3\n", -3),
+    ("1\n2\nx\n4\n", -9),
+])
+def test_tour_read_cases(tmp_path, text, code):
+    p = tmp_path / "t.tour"
+    p.write_text(text)
+    if code:
+        with pytest.raises(L2.Logo2OptError) as e:
+            L2.tour_read(str(p), 4)
+        assert e.value.code == code
+    else:
+        assert L2.tour_read(str(p), 4).tolist() == [0, 1, 2, 3]
+
+
+def test_tour_write_rejects_non_permutation(tmp_path):
+    with pytest.raises(L2.Logo2OptError) as e:
+        L2.tour_write(str(tmp_path / "x.tour"), np.array([0, 1, 1, 3], np.uint32))
+    assert e.value.code == -3
