@@ -286,6 +286,13 @@ void logo2opt_random_perm(uint32_t n, uint64_t seed, uint32_t *route);
  * lowest id on ties) on a uniform grid; replaces routeInit's NN option
  * (construction.c:139-142, kdtree_utils.c:279-301). */
 int  logo2opt_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route);
+/* Multiple-fragment (greedy edge) tour, the reference's default construction
+ * (defs.h:70 "mf"; multipleFragment_KD, kdtree_utils.c:56-248): edges in
+ * increasing (squared length, lower id, higher id) order join cities of degree
+ * < 2 in different fragments; the final path is closed.  Exact integer
+ * arithmetic, duplicates kept (the reference's kd-tree drops them).
+ * route[0] = 0, route[1] = the lower-id neighbour of city 0. */
+int  logo2opt_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route);
 /* int32 xy -> reference city_coords */
 void logo2opt_xy_to_coords(uint32_t n, const int32_t *xy, logo2opt_city_coords *coords);
 

@@ -1,5 +1,5 @@
 // l2o_workload.cpp -- deterministic synthetic workloads and the greedy
-// nearest-neighbour construction (host C++).
+// nearest-neighbour and multiple-fragment constructions (host C++).
 //
 // The reference builds its inputs from TSPLIB files and time-seeded libc
 // rand() (construction.c:118-150, common.c:253-259); the benchmark configs
@@ -9,6 +9,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
+#include <numeric>
+#include <queue>
 #include <vector>
 
 #include "../../include/logo2opt.h"
@@ -49,6 +52,89 @@ int32_t round_clamp(double v, uint32_t side)
     if (r > (double)(side - 1)) return (int32_t)(side - 1);
     return (int32_t)r;
 }
+
+// Points bucketed on a uniform G x G grid (about `per_cell` points per cell)
+// with O(1) removal, for exact nearest-point queries.
+struct point_grid {
+    const int32_t *xy = nullptr;
+    int64_t mnx = 0, mny = 0, cw = 1, G = 1;
+    std::vector<std::vector<uint32_t>> cells;
+    std::vector<uint32_t> pos;
+
+    void build(uint32_t n, const int32_t *p, double per_cell)
+    {
+        xy = p;
+        int64_t mxx = p[0], mxy = p[1];
+        mnx = p[0];
+        mny = p[1];
+        for (uint32_t k = 1; k < n; ++k) {
+            mnx = std::min<int64_t>(mnx, p[2 * k]);
+            mxx = std::max<int64_t>(mxx, p[2 * k]);
+            mny = std::min<int64_t>(mny, p[2 * k + 1]);
+            mxy = std::max<int64_t>(mxy, p[2 * k + 1]);
+        }
+        const int64_t span = std::max<int64_t>(1, std::max(mxx - mnx, mxy - mny) + 1);
+        G = (int64_t)std::ceil(std::sqrt((double)n / per_cell));
+        G = std::max<int64_t>(1, std::min<int64_t>(G, 4096));
+        cw = (span + G - 1) / G;  // G * cw >= span
+        cells.assign((size_t)(G * G), {});
+        pos.assign(n, 0);
+        for (uint32_t c = 0; c < n; ++c) {
+            auto &v = cells[cell(c)];
+            pos[c] = (uint32_t)v.size();
+            v.push_back(c);
+        }
+    }
+    size_t cell(uint32_t c) const
+    {
+        return (size_t)(((xy[2 * c + 1] - mny) / cw) * G + (xy[2 * c] - mnx) / cw);
+    }
+    void remove(uint32_t c)
+    {
+        auto &v = cells[cell(c)];
+        const uint32_t p = pos[c], last = v.back();
+        v[p] = last;
+        pos[last] = p;
+        v.pop_back();
+    }
+    // The point u != c in the grid with ok(u) that minimises (squared
+    // distance, u); UINT32_MAX if there is none.  Rings of cells are searched
+    // outward: ring R holds no point closer than (R-1) cell widths, so the
+    // search ends once that bound exceeds the best distance (equal distances
+    // in farther rings are still examined for the id tie-break).
+    uint32_t nearest(uint32_t c, const std::function<bool(uint32_t)> &ok, uint64_t *s_out) const
+    {
+        const int64_t px = xy[2 * c], py = xy[2 * c + 1];
+        const int64_t gx = (px - mnx) / cw, gy = (py - mny) / cw;
+        uint64_t best = UINT64_MAX;
+        uint32_t arg = UINT32_MAX;
+        for (int64_t R = 0; R < G; ++R) {
+            if (R >= 2 && best != UINT64_MAX) {
+                const uint64_t lb = (uint64_t)((R - 1) * cw) * (uint64_t)((R - 1) * cw);
+                if (lb > best) break;
+            }
+            const int64_t y0 = std::max<int64_t>(0, gy - R), y1 = std::min<int64_t>(G - 1, gy + R);
+            for (int64_t y = y0; y <= y1; ++y) {
+                const bool full = (y == gy - R || y == gy + R);
+                const int64_t step = full ? 1 : 2 * R;
+                for (int64_t x = gx - R; x <= gx + R; x += (step > 0 ? step : 1)) {
+                    if (x < 0 || x >= G) continue;
+                    for (uint32_t u : cells[(size_t)(y * G + x)]) {
+                        if (u == c) continue;
+                        const int64_t dx = xy[2 * u] - px, dy = xy[2 * u + 1] - py;
+                        const uint64_t s = (uint64_t)(dx * dx + dy * dy);
+                        if ((s < best || (s == best && u < arg)) && ok(u)) {
+                            best = s;
+                            arg = u;
+                        }
+                    }
+                }
+            }
+        }
+        *s_out = best;
+        return arg;
+    }
+};
 
 }  // namespace
 
@@ -187,6 +273,90 @@ int logo2opt_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *ro
         cur = arg;
     }
     return LOGO2OPT_OK;
+}
+
+// Multiple-fragment (greedy edge) construction, the reference's default
+// initial tour (defs.h:70 "mf"; multipleFragment_KD, kdtree_utils.c:56-248):
+// edges are added in increasing order of (squared length, lower id, higher id)
+// whenever both ends have degree < 2 and they lie in different fragments,
+// until one Hamiltonian path remains, which is closed.  As in the reference,
+// each city with degree < 2 keeps its nearest valid neighbour in a priority
+// queue and is re-queried when that candidate has become invalid (a valid
+// edge's key can only grow, so the queue's top, once valid, is the smallest
+// valid edge).  The reference keeps the candidates in a kd-tree over float
+// coordinates that drops duplicate points (kdtree_utils.c:42-48); here a grid
+// over the integer coordinates answers exact queries, so the result is the
+// greedy-edge tour for the (squared Euclidean, id) order, duplicates
+// included.  The squared length orders EUC_2D, CEIL_2D and ATT edges alike
+// (all are non-decreasing in it).  route[0] = 0, route[1] = the lower-id
+// neighbour of city 0.
+int logo2opt_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route)
+{
+    if (n < 3 || n > LOGO2OPT_MAX_N || !xy || !route) return LOGO2OPT_EINVAL;
+    point_grid g;
+    g.build(n, xy, 2.0);
+    std::vector<uint8_t> deg(n, 0);
+    std::vector<uint32_t> uf(n), adj(2 * (size_t)n, UINT32_MAX);
+    std::iota(uf.begin(), uf.end(), 0u);
+    auto find = [&](uint32_t v) {
+        while (uf[v] != v) {
+            uf[v] = uf[uf[v]];
+            v = uf[v];
+        }
+        return v;
+    };
+    struct cand {
+        uint64_t s;
+        uint32_t lo, hi, x, y;  // edge (lo, hi) = {x, y}, found for x
+        bool operator>(const cand &o) const
+        {
+            if (s != o.s) return s > o.s;
+            if (lo != o.lo) return lo > o.lo;
+            if (hi != o.hi) return hi > o.hi;
+            return x > o.x;
+        }
+    };
+    std::priority_queue<cand, std::vector<cand>, std::greater<cand>> pq;
+    auto query = [&](uint32_t x) {
+        const uint32_t rx = find(x);
+        uint64_t s;
+        const uint32_t y = g.nearest(x, [&](uint32_t u) { return find(u) != rx; }, &s);
+        if (y != UINT32_MAX) pq.push(cand{s, std::min(x, y), std::max(x, y), x, y});
+    };
+    for (uint32_t x = 0; x < n; ++x) query(x);
+    uint32_t added = 0;
+    while (added + 1 < n) {
+        if (pq.empty()) return LOGO2OPT_ESTATE;  // cannot happen while two fragments remain
+        const cand c = pq.top();
+        pq.pop();
+        if (deg[c.x] == 2) continue;
+        if (deg[c.y] < 2 && find(c.x) != find(c.y)) {
+            for (uint32_t v : {c.x, c.y}) {
+                adj[2 * (size_t)v + deg[v]] = v == c.x ? c.y : c.x;
+                if (++deg[v] == 2) g.remove(v);
+            }
+            uf[find(c.x)] = find(c.y);
+            ++added;
+            if (deg[c.x] < 2) query(c.x);
+        } else {
+            query(c.x);  // stale candidate: find x's nearest valid neighbour again
+        }
+    }
+    // close the path: its two ends have degree 1
+    uint32_t e[2], ne = 0;
+    for (uint32_t v = 0; v < n && ne < 2; ++v)
+        if (deg[v] < 2) e[ne++] = v;
+    if (ne != 2) return LOGO2OPT_ESTATE;
+    for (uint32_t v : {e[0], e[1]}) adj[2 * (size_t)v + deg[v]++] = v == e[0] ? e[1] : e[0];
+    uint32_t prev = 0, cur = std::min(adj[0], adj[1]);
+    route[0] = 0;
+    for (uint32_t k = 1; k < n; ++k) {
+        route[k] = cur;
+        const uint32_t nx = adj[2 * (size_t)cur] == prev ? adj[2 * (size_t)cur + 1] : adj[2 * (size_t)cur];
+        prev = cur;
+        cur = nx;
+    }
+    return cur == 0 ? LOGO2OPT_OK : LOGO2OPT_ESTATE;
 }
 
 }  // extern "C"

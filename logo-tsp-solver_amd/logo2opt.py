@@ -33,7 +33,7 @@ EXPORTS = [
     "logo2opt_random_perm", "logo2opt_nn_tour", "logo2opt_xy_to_coords", "logo2opt_sqrt_ulp_check",
     "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
     "logo2opt_engine_time_sweep", "logo2opt_create_metric", "logo2opt_tsp_read", "logo2opt_tour_read",
-    "logo2opt_tour_write", "logo2opt_host_route_length",
+    "logo2opt_tour_write", "logo2opt_host_route_length", "logo2opt_mf_tour",
 ]
 
 
@@ -130,6 +130,7 @@ def load():
     L.logo2opt_random_perm.argtypes = [C.c_uint32, C.c_uint64, u32p]
     L.logo2opt_random_perm.restype = None
     L.logo2opt_nn_tour.argtypes = [C.c_uint32, i32p, C.c_uint32, u32p]
+    L.logo2opt_mf_tour.argtypes = [C.c_uint32, i32p, u32p]
     L.logo2opt_xy_to_coords.argtypes = [C.c_uint32, i32p, vp]
     L.logo2opt_xy_to_coords.restype = None
     L.logo2opt_sqrt_ulp_check.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
@@ -174,6 +175,14 @@ def nn_tour(xy, start=0):
     xy = np.ascontiguousarray(xy, np.int32)
     r = np.zeros(len(xy) // 2, np.uint32)
     _check(load().logo2opt_nn_tour(len(r), xy, start, r), "nn_tour")
+    return r
+
+
+def mf_tour(xy):
+    """Multiple-fragment (greedy edge) tour of int32 interleaved xy."""
+    xy = np.ascontiguousarray(xy, np.int32)
+    r = np.zeros(len(xy) // 2, np.uint32)
+    _check(load().logo2opt_mf_tour(len(r), xy, r), "mf_tour")
     return r
 
 

@@ -151,6 +151,80 @@ void orc_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route)
     free(used);
 }
 
+/* Greedy edge (multiple fragment) by brute force: all n(n-1)/2 edges sorted
+ * by (squared length, lower id, higher id), each accepted when both ends have
+ * degree < 2 and lie in different fragments (union-find); the remaining path
+ * is closed.  Same selection rule as kdtree_utils.c:56-248 without its
+ * kd-tree (which drops duplicate points).  O(n^2 log n): small n only.
+ * route[0] = 0, route[1] = the lower-id neighbour of city 0.  Returns 0, or
+ * -1 on n < 3 / allocation failure. */
+typedef struct { uint64_t s; uint32_t a, b; } mf_edge;
+
+static int mf_cmp(const void *p, const void *q)
+{
+    const mf_edge *x = (const mf_edge *)p, *y = (const mf_edge *)q;
+    if (x->s != y->s) return x->s < y->s ? -1 : 1;
+    if (x->a != y->a) return x->a < y->a ? -1 : 1;
+    if (x->b != y->b) return x->b < y->b ? -1 : 1;
+    return 0;
+}
+
+static uint32_t uf_find(uint32_t *uf, uint32_t v)
+{
+    while (uf[v] != v) v = uf[v] = uf[uf[v]];
+    return v;
+}
+
+int orc_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route)
+{
+    if (n < 3) return -1;
+    uint64_t m = (uint64_t)n * (n - 1) / 2, k = 0;
+    mf_edge *e = (mf_edge *)malloc(sizeof(mf_edge) * m);
+    uint32_t *uf = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *adj = (uint32_t *)malloc(sizeof(uint32_t) * 2 * n);
+    unsigned char *deg = (unsigned char *)calloc(n, 1);
+    if (!e || !uf || !adj || !deg) { free(e); free(uf); free(adj); free(deg); return -1; }
+    for (uint32_t a = 0; a < n; a++)
+        for (uint32_t b = a + 1; b < n; b++) {
+            int64_t dx = (int64_t)xy[2 * a] - xy[2 * b], dy = (int64_t)xy[2 * a + 1] - xy[2 * b + 1];
+            e[k].s = (uint64_t)(dx * dx + dy * dy);
+            e[k].a = a;
+            e[k].b = b;
+            k++;
+        }
+    qsort(e, m, sizeof(mf_edge), mf_cmp);
+    for (uint32_t v = 0; v < n; v++) uf[v] = v;
+    uint32_t added = 0;
+    for (uint64_t q = 0; q < m && added + 1 < n; q++) {
+        uint32_t a = e[q].a, b = e[q].b;
+        if (deg[a] == 2 || deg[b] == 2) continue;
+        uint32_t ra = uf_find(uf, a), rb = uf_find(uf, b);
+        if (ra == rb) continue;
+        adj[2 * a + deg[a]++] = b;
+        adj[2 * b + deg[b]++] = a;
+        uf[ra] = rb;
+        added++;
+    }
+    uint32_t ends[2], ne = 0;
+    for (uint32_t v = 0; v < n && ne < 2; v++)
+        if (deg[v] < 2) ends[ne++] = v;
+    adj[2 * ends[0] + deg[ends[0]]++] = ends[1];
+    adj[2 * ends[1] + deg[ends[1]]++] = ends[0];
+    uint32_t prev = 0, cur = adj[0] < adj[1] ? adj[0] : adj[1];
+    route[0] = 0;
+    for (uint32_t i = 1; i < n; i++) {
+        route[i] = cur;
+        uint32_t nx = adj[2 * cur] == prev ? adj[2 * cur + 1] : adj[2 * cur];
+        prev = cur;
+        cur = nx;
+    }
+    free(e);
+    free(uf);
+    free(adj);
+    free(deg);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Distances                                                                  */
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,8 @@ void orc_random_perm(uint32_t n, uint64_t seed, uint32_t *route);
 /* greedy nearest neighbour from `start`, exact int64 squared distance,
  * lowest city id on ties; O(n^2) */
 void orc_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route);
+/* greedy edge (multiple fragment) by sorting all edges; O(n^2 log n) */
+int  orc_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route);
 
 /* ---- distances -------------------------------------------------------- */
 /* nint(sqrt(s)) = k + [s > k^2 + k] with k = isqrt(s) */

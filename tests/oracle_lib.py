@@ -36,6 +36,7 @@ def _load_oracle():
                                     C.c_uint64, i32p]
     L.orc_random_perm.argtypes = [C.c_uint32, C.c_uint64, u32p]
     L.orc_nn_tour.argtypes = [C.c_uint32, i32p, C.c_uint32, u32p]
+    L.orc_mf_tour.argtypes = [C.c_uint32, i32p, u32p]
     L.orc_nint_sqrt.argtypes = [C.c_uint64]
     L.orc_nint_sqrt.restype = C.c_uint32
     L.orc_nint_sqrt_batch.argtypes = [u64p, u32p, C.c_uint64]
@@ -99,6 +100,13 @@ def nn_tour(xy, start=0):
     n = len(xy) // 2
     r = np.zeros(n, np.uint32)
     lib.orc_nn_tour(n, xy, start, r)
+    return r
+
+
+def mf_tour(xy):
+    n = len(xy) // 2
+    r = np.zeros(n, np.uint32)
+    assert lib.orc_mf_tour(n, xy, r) == 0
     return r
 
 
