@@ -121,6 +121,20 @@ int logo2opt_optimize(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t *length
 int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds,
                  int kick, uint64_t *best_length, uint64_t *round_lengths);
 
+/* ILSGlobalSolverMT::optimize with comm = 1 and `chains` threads
+ * (ILSGlobalSolverMT.cpp:86-195): chains of ILS on the same instance that
+ * share a global best, here race free.  All chains start from route_inout and
+ * advance round by round on the device; after every round the group best is
+ * the smallest best length (lowest chain on ties); a chain whose local search
+ * improved its own best kicks that tour next, any other chain kicks the group
+ * best (the reference's "route = bestGlobalMinima.back()").  Chain c draws
+ * its kicks from seed + c.  route_inout receives the group best tour,
+ * round_lengths (optional, rounds+1 entries) the group best after the first
+ * local search and after every round.  The result does not depend on timing.
+ * rounds >= 1, 1 <= chains <= 4096. */
+int logo2opt_ils_multistart(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds,
+                            uint32_t chains, int kick, uint64_t *best_length, uint64_t *round_lengths);
+
 /* routeLength (src/headers.h:170-189), exact, computed on the device. */
 int logo2opt_route_length(logo2opt_ctx *ctx, const uint32_t *route, uint64_t *length);
 
@@ -200,6 +214,20 @@ int logo2opt_engine_result(logo2opt_engine *e, int32_t slot, uint32_t *route_out
 int logo2opt_engine_round_lengths(logo2opt_engine *e, int32_t slot, uint64_t *out, uint32_t cap);
 /* Free a slot's device memory (finished or not). */
 int logo2opt_engine_remove(logo2opt_engine *e, int32_t slot);
+/* Multi-start ILS group (see logo2opt_ils_multistart): `chains` chains of one
+ * instance in consecutive slots first_slot .. first_slot+chains-1; job must
+ * ask for ILS (ils_rounds >= 1); chain c uses seed job->seed + c.  Each slot
+ * reports its own chain through engine_result / round_lengths (the group best
+ * per round); removing any chain removes the whole group.  EAGAIN when no
+ * run of `chains` free slots or not enough HBM budget is left. */
+int logo2opt_engine_add_group(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords,
+                              const uint32_t *route, const logo2opt_job *job, uint32_t chains,
+                              int32_t *first_slot);
+/* The group best (route_out: its tour; res: its length, moves/sweeps/
+ * fallbacks summed over the chains, done when every chain is) and, in
+ * best_chain, which chain holds it. */
+int logo2opt_engine_group_result(logo2opt_engine *e, int32_t first_slot, uint32_t *route_out,
+                                 logo2opt_result *res, int32_t *best_chain);
 /* Slots finished since the last call (ids written to out, at most cap). */
 int logo2opt_engine_poll_done(logo2opt_engine *e, int32_t *out, uint32_t cap, uint32_t *count);
 int logo2opt_engine_stats(logo2opt_engine *e, logo2opt_stats *st);

@@ -199,6 +199,8 @@ struct slot_t {
     bool done = false;
     bool done_reported = false;
     float slack = 1.0f;
+    uint32_t group_lead = L2O_NO_GROUP;  // multi-start ILS group (first slot)
+    uint32_t group_size = 0;
 };
 
 }  // namespace
@@ -228,8 +230,9 @@ struct logo2opt_engine {
     l2o_apply_item *d_aitems = nullptr;
     size_t aitems_cap = 0;
     uint32_t n_aitems = 0;
-    uint32_t ils_grid = 0;
+    uint32_t ils_grid = 0;    // k_ils covers slots [0, ils_grid)
     bool any_ils = false;
+    uint32_t group_grid = 0;  // k_group covers slots [0, group_grid)
     bool dirty = true;
     unsigned live_at_build = 0;
 
@@ -253,6 +256,8 @@ struct logo2opt_engine {
 
 struct logo2opt_ctx {
     logo2opt_engine *e = nullptr;
+    logo2opt_engine *ge = nullptr;  // multi-start ILS groups (created on first use)
+    uint32_t ge_chains = 0;
     uint32_t n = 0;
     uint32_t metric = LOGO2OPT_METRIC_EUC_2D;
     std::vector<logo2opt_city_coords> coords;
@@ -346,6 +351,9 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
     in.ils_rounds = s.job.ils_rounds;
     in.kick = s.job.kick;
     in.metric = s.job.metric;
+    in.group_lead = s.group_lead;
+    in.group_size = s.group_size;
+    in.kick_src = (uint32_t)id;
     in.est_scale = s.job.metric == l2o::L2O_ATT ? (float)(1.0 / std::sqrt(10.0)) : 1.0f;
     in.est_add = s.job.metric == l2o::L2O_EUC_2D ? 0.0f : 0.5f;
     in.phase = phase;
@@ -412,6 +420,7 @@ int rebuild_lists(logo2opt_engine *e)
     std::vector<l2o_apply_item> ai;
     e->ils_grid = 0;
     e->any_ils = false;
+    e->group_grid = 0;
     // largest instances first: their long sweeps start early and the small
     // ones fill the tail (longest-processing-time order)
     std::vector<uint32_t> order;
@@ -433,8 +442,9 @@ int rebuild_lists(logo2opt_engine *e)
         for (uint32_t ch = 0; ch < nch; ++ch) ai.push_back({id, ch});
         if (s.job.ils_rounds) {
             e->any_ils = true;
-            e->ils_grid = id + 1;
+            e->ils_grid = std::max(e->ils_grid, id + 1);  // `order` is not by id
         }
+        if (s.group_lead != L2O_NO_GROUP) e->group_grid = std::max(e->group_grid, id + 1);
     }
     std::vector<l2o_sweep_item> all;
     e->groups.clear();
@@ -522,6 +532,7 @@ int issue_step(logo2opt_engine *e)
     }
     L2O_TRY(l2o::launch_apply(e->d_aitems, e->n_aitems, e->d_insts, e->d_live, e->stream));
     if (e->any_ils) L2O_TRY(l2o::launch_ils(e->d_insts, e->ils_grid, e->d_live, e->stream));
+    if (e->group_grid) L2O_TRY(l2o::launch_group(e->d_insts, e->group_grid, e->d_live, e->stream));
     return LOGO2OPT_OK;
 }
 
@@ -618,8 +629,9 @@ int engine_run(logo2opt_engine *e, uint32_t max_steps, uint32_t sync_every, uint
     return LOGO2OPT_OK;
 }
 
-int engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
-               const logo2opt_job *job, int32_t *slot_out)
+// admits an instance into slot `id` (free); group_lead / group_size as in l2o_inst
+int engine_add_at(logo2opt_engine *e, int32_t id, uint32_t n, const logo2opt_city_coords *coords,
+                  const uint32_t *route, const logo2opt_job *job, uint32_t group_lead, uint32_t group_size)
 {
     if (n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
     slot_t tmp;
@@ -627,13 +639,8 @@ int engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coord
     if (rc) return rc;
     rc = check_tour(n, route);
     if (rc) return rc;
-    int32_t id = -1;
-    for (uint32_t k = 0; k < e->max_slots; ++k)
-        if (!e->slots[k].used) {
-            id = (int32_t)k;
-            break;
-        }
-    if (id < 0) return LOGO2OPT_EAGAIN;
+    tmp.group_lead = group_lead;
+    tmp.group_size = group_size;
     logo2opt_job j{};
     if (job) j = *job;
     if (j.kick > 1 || j.metric > LOGO2OPT_METRIC_ATT) return LOGO2OPT_EINVAL;
@@ -661,7 +668,51 @@ int engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coord
     L2O_TRY(hipMemcpyAsync(e->d_live, e->h_live, sizeof(unsigned), hipMemcpyHostToDevice, e->stream));
     L2O_TRY(hipStreamSynchronize(e->stream));
     e->dirty = true;
-    *slot_out = id;
+    return LOGO2OPT_OK;
+}
+
+int engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
+               const logo2opt_job *job, int32_t *slot_out)
+{
+    int32_t id = -1;
+    for (uint32_t k = 0; k < e->max_slots; ++k)
+        if (!e->slots[k].used) {
+            id = (int32_t)k;
+            break;
+        }
+    if (id < 0) return LOGO2OPT_EAGAIN;
+    const int rc = engine_add_at(e, id, n, coords, route, job, L2O_NO_GROUP, 0);
+    if (rc == LOGO2OPT_OK) *slot_out = id;
+    return rc;
+}
+
+int engine_remove(logo2opt_engine *e, int32_t id);
+
+// `chains` ILS chains of one instance in consecutive slots; chain c uses the
+// seed job->seed + c
+int engine_add_group(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
+                     const logo2opt_job *job, uint32_t chains, int32_t *first_out)
+{
+    if (!job || job->ils_rounds == 0 || chains == 0 || chains > e->max_slots) return LOGO2OPT_EINVAL;
+    int32_t first = -1;
+    for (uint32_t k = 0; k + chains <= e->max_slots && first < 0; ++k) {
+        uint32_t c = 0;
+        while (c < chains && !e->slots[k + c].used) ++c;
+        if (c == chains) first = (int32_t)k;
+        else k += c;  // slot k + c is used: resume after it
+    }
+    if (first < 0) return LOGO2OPT_EAGAIN;
+    for (uint32_t c = 0; c < chains; ++c) {
+        logo2opt_job j = *job;
+        j.seed = job->seed + c;
+        const int rc = engine_add_at(e, first + (int32_t)c, n, coords, route, &j, (uint32_t)first, chains);
+        if (rc) {
+            for (uint32_t q = 0; q < c; ++q) e->slots[first + q].group_lead = L2O_NO_GROUP;
+            for (uint32_t q = 0; q < c; ++q) (void)engine_remove(e, first + (int32_t)q);
+            return rc;
+        }
+    }
+    *first_out = first;
     return LOGO2OPT_OK;
 }
 
@@ -695,6 +746,19 @@ int engine_result(logo2opt_engine *e, int32_t id, uint32_t *route_out, logo2opt_
 int engine_remove(logo2opt_engine *e, int32_t id)
 {
     if (id < 0 || (uint32_t)id >= e->max_slots || !e->slots[id].used) return LOGO2OPT_ENOTFOUND;
+    {
+        // a chain of a multi-start group cannot run on without the others:
+        // the whole group is retired
+        const uint32_t lead = e->slots[id].group_lead;
+        if (lead != L2O_NO_GROUP) {
+            const uint32_t m = e->slots[lead].group_size;
+            for (uint32_t c = 0; c < m; ++c) e->slots[lead + c].group_lead = L2O_NO_GROUP;
+            int rc = LOGO2OPT_OK;
+            for (uint32_t c = 0; c < m; ++c)
+                if (e->slots[lead + c].used && rc == LOGO2OPT_OK) rc = engine_remove(e, (int32_t)(lead + c));
+            return rc;
+        }
+    }
     slot_t &s = e->slots[id];
     if (!s.done) {
         // retire a running instance: mark it done on the device first
@@ -870,6 +934,48 @@ int logo2opt_engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coor
     return engine_add(e, n, coords, route, job, slot);
 }
 
+int logo2opt_engine_add_group(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords,
+                              const uint32_t *route, const logo2opt_job *job, uint32_t chains, int32_t *first_slot)
+{
+    if (!e || !first_slot) return LOGO2OPT_EINVAL;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    return engine_add_group(e, n, coords, route, job, chains, first_slot);
+}
+
+int logo2opt_engine_group_result(logo2opt_engine *e, int32_t first_slot, uint32_t *route_out, logo2opt_result *res,
+                                 int32_t *best_chain)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    if (first_slot < 0 || (uint32_t)first_slot >= e->max_slots || !e->slots[first_slot].used ||
+        e->slots[first_slot].group_lead != (uint32_t)first_slot)
+        return LOGO2OPT_ENOTFOUND;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    const uint32_t m = e->slots[first_slot].group_size;
+    std::vector<l2o_inst> h(m);
+    L2O_TRY(hipMemcpyAsync(h.data(), e->d_insts + first_slot, sizeof(l2o_inst) * m, hipMemcpyDeviceToHost,
+                           e->stream));
+    L2O_TRY(hipStreamSynchronize(e->stream));
+    uint32_t b = 0;
+    for (uint32_t c = 1; c < m; ++c)
+        if (h[c].have_best && (!h[b].have_best || h[c].best_length < h[b].best_length)) b = c;
+    logo2opt_result r;
+    int rc = engine_result(e, first_slot + (int32_t)b, route_out, &r, nullptr);
+    if (rc) return rc;
+    if (res) {
+        *res = r;
+        res->moves = res->sweeps = res->fallbacks = 0;
+        res->done = 1;
+        for (uint32_t c = 0; c < m; ++c) {
+            res->moves += h[c].moves;
+            res->sweeps += h[c].sweeps;
+            res->fallbacks += h[c].fallbacks;
+            res->done &= h[c].phase == L2O_DONE ? 1u : 0u;
+        }
+    }
+    if (best_chain) *best_chain = (int32_t)b;
+    return LOGO2OPT_OK;
+}
+
 int logo2opt_engine_run(logo2opt_engine *e, uint32_t max_steps, uint32_t sync_every, uint32_t *live)
 {
     if (!e) return LOGO2OPT_EINVAL;
@@ -989,6 +1095,7 @@ int logo2opt_destroy(logo2opt_ctx *ctx)
 {
     if (!ctx) return LOGO2OPT_EINVAL;
     if (ctx->e) logo2opt_engine_destroy(ctx->e);
+    if (ctx->ge) logo2opt_engine_destroy(ctx->ge);
     delete ctx;
     return LOGO2OPT_OK;
 }
@@ -1120,6 +1227,52 @@ int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32
         if (rc) return rc;
     }
     return LOGO2OPT_OK;
+}
+
+int logo2opt_ils_multistart(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds,
+                            uint32_t chains, int kick, uint64_t *best_length, uint64_t *round_lengths)
+{
+    if (!ctx || !route_inout || (kick != 0 && kick != 1) || rounds == 0 || chains == 0 || chains > 4096)
+        return LOGO2OPT_EINVAL;
+    if (hipSetDevice(ctx->e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    logo2opt_job job{};
+    job.ils_rounds = rounds;
+    job.kick = (uint32_t)kick;
+    job.seed = seed;
+    job.metric = ctx->metric;
+    int rc;
+    if (ctx->ge && (ctx->ge_chains < chains || ctx->ge->ar.size < chains * bytes_for(ctx->n, &job))) {
+        logo2opt_engine_destroy(ctx->ge);
+        ctx->ge = nullptr;
+    }
+    if (!ctx->ge) {
+        rc = engine_create(ctx->e->device, chains * bytes_for(ctx->n, &job) + (1ull << 24), chains, &ctx->ge);
+        if (rc) return rc;
+        ctx->ge_chains = chains;
+    }
+    logo2opt_engine *e = ctx->ge;
+    for (uint32_t k = 0; k < e->max_slots; ++k)
+        if (e->slots[k].used) {
+            rc = engine_remove(e, (int32_t)k);
+            if (rc) return rc;
+        }
+    int32_t first = -1;
+    rc = engine_add_group(e, ctx->n, ctx->coords.data(), route_inout, &job, chains, &first);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    uint32_t live = 1;
+    rc = engine_run(e, 0xFFFFFFFFu, 64, &live);
+    if (rc) return rc;
+    ctx->wall_ms += now_ms() - t0;
+    logo2opt_result res;
+    rc = logo2opt_engine_group_result(e, first, route_inout, &res, nullptr);
+    if (rc) return rc;
+    ctx->sweeps += res.sweeps;
+    ctx->moves += res.moves;
+    ctx->fallbacks += res.fallbacks;
+    if (best_length) *best_length = res.length;
+    if (round_lengths) rc = logo2opt_engine_round_lengths(e, first, round_lengths, rounds + 1);
+    return rc;
 }
 
 int logo2opt_engine_round_lengths(logo2opt_engine *e, int32_t slot, uint64_t *out, uint32_t cap)

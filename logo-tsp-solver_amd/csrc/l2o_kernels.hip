@@ -515,25 +515,41 @@ __device__ unsigned long long xoshiro_next(unsigned long long *s)
     return result;
 }
 
+// A plain ILS instance at L2O_LO_DONE: accept iff strictly shorter (copy to
+// the best buffers), stop after `rounds`, else kick its best and restart.
+// A chain of a multi-start group at L2O_LO_DONE only accepts into its own best
+// and waits (L2O_WAIT) for k_group; at L2O_KICK it kicks the best tour of
+// `kick_src` that k_group chose.  While any chain of a group is at L2O_KICK,
+// all of them are, so no best buffer is written while a kick reads it.
 __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ insts, unsigned *__restrict__ live)
 {
     l2o_inst *in = insts + blockIdx.x;
-    if (in->phase != L2O_LO_DONE) return;
+    const uint32_t phase = in->phase;
+    if (phase != L2O_LO_DONE && phase != L2O_KICK) return;
+    const bool grouped = in->group_lead != L2O_NO_GROUP;
     __shared__ int s_accept, s_done;
     __shared__ uint32_t s_a, s_b, s_c;
     __shared__ unsigned long long s_sum[L2O_ILS_THREADS / 64];
     const uint32_t n = in->n;
     const uint32_t par = in->parity;
     if (threadIdx.x == 0) {
-        const int acc = !in->have_best || in->length < in->best_length;
-        s_accept = acc;
-        if (acc) {
-            in->best_length = in->length;
-            in->have_best = 1;
+        int acc = 0, done = 0;
+        if (phase == L2O_LO_DONE) {
+            acc = !in->have_best || in->length < in->best_length;
+            if (acc) {
+                in->best_length = in->length;
+                in->have_best = 1;
+            }
+            if (grouped) {
+                in->improved = acc;
+            } else {
+                if (in->rlog) in->rlog[in->rounds_done] = in->best_length;
+                done = in->rounds_done >= in->ils_rounds;
+            }
         }
-        if (in->rlog) in->rlog[in->rounds_done] = in->best_length;
-        s_done = in->rounds_done >= in->ils_rounds;
-        if (!s_done) {
+        s_accept = acc;
+        s_done = done;
+        if (!done && (phase == L2O_KICK || !grouped)) {
             unsigned long long *rng = in->rng;
             if (in->kick == 1) {  // reference kick: a, b = rand() % n, sorted (ILSGlobalSolver.cpp:39-44)
                 uint32_t a = (uint32_t)(xoshiro_next(rng) % n);
@@ -565,17 +581,24 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
         }
     }
     __syncthreads();
-    const l2o_pt *__restrict__ P = in->pts[par];
-    const uint32_t *__restrict__ R = in->tour[par];
-    l2o_pt *__restrict__ BP = in->bpts;
-    uint32_t *__restrict__ BR = in->btour;
     if (s_accept) {
+        const l2o_pt *__restrict__ P = in->pts[par];
+        const uint32_t *__restrict__ R = in->tour[par];
+        l2o_pt *__restrict__ BP = in->bpts;
+        uint32_t *__restrict__ BR = in->btour;
         for (uint32_t k = threadIdx.x; k < n; k += L2O_ILS_THREADS) {
             BP[k] = P[k];
             BR[k] = R[k];
         }
     }
     __syncthreads();
+    if (grouped && phase == L2O_LO_DONE) {
+        if (threadIdx.x == 0) {
+            __threadfence();
+            in->phase = L2O_WAIT;
+        }
+        return;
+    }
     if (s_done) {
         if (threadIdx.x == 0) {
             in->phase = L2O_DONE;
@@ -583,7 +606,10 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
         }
         return;
     }
-    // new current tour = kick(best), written to buffer 0
+    // new current tour = kick(best of the source), written to buffer 0
+    const l2o_inst *src = grouped ? insts + in->kick_src : in;
+    const l2o_pt *__restrict__ BP = src->bpts;
+    const uint32_t *__restrict__ BR = src->btour;
     const uint32_t a = s_a, b = s_b, c = s_c;
     const bool ref = in->kick == 1;
     auto src_of = [&](uint32_t k) -> uint32_t {
@@ -634,6 +660,43 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
         __threadfence();
         in->phase = L2O_LO;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-start ILS round barrier: ILSGlobalSolverMT::optimize with comm = 1
+// (ILSGlobalSolverMT.cpp:86-195) shares a global best between threads under a
+// mutex, so its result depends on timing.  Here the chains of a group advance
+// round by round: once every chain has reached its local optimum (L2O_WAIT),
+// one thread per group takes the group best (smallest best length, lowest
+// chain on ties), logs it, and sends each chain on with a kick of its own
+// tour if that improved its best this round, else of the group best (the
+// reference's "route = bestGlobalMinima.back()"); after `rounds` kicks every
+// chain stops.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_group(l2o_inst *__restrict__ insts, uint32_t nslots,
+                                              unsigned *__restrict__ live)
+{
+    const uint32_t id = blockIdx.x * 64 + threadIdx.x;
+    if (id >= nslots) return;
+    l2o_inst *lead = insts + id;
+    if (lead->group_lead != id || lead->phase != L2O_WAIT) return;
+    const uint32_t m = lead->group_size;
+    uint32_t best = id;
+    for (uint32_t c = 0; c < m; ++c) {
+        const l2o_inst *in = insts + id + c;
+        if (in->phase != L2O_WAIT) return;  // not all chains are done with this round
+        if (in->best_length < insts[best].best_length) best = id + c;
+    }
+    const unsigned long long G = insts[best].best_length;
+    const uint32_t rounds = lead->rounds_done;
+    const bool done = rounds >= lead->ils_rounds;
+    for (uint32_t c = 0; c < m; ++c) {
+        l2o_inst *in = insts + id + c;
+        if (in->rlog) in->rlog[rounds] = G;
+        in->kick_src = in->improved ? id + c : best;
+        in->phase = done ? L2O_DONE : L2O_KICK;
+    }
+    if (done) atomicSub(live, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -747,6 +810,13 @@ hipError_t launch_ils(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStrea
 {
     if (nslots == 0) return hipSuccess;
     hipLaunchKernelGGL(k_ils, dim3(nslots), dim3(L2O_ILS_THREADS), 0, stream, insts, live);
+    return hipGetLastError();
+}
+
+hipError_t launch_group(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStream_t stream)
+{
+    if (nslots == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_group, dim3((nslots + 63) / 64), dim3(64), 0, stream, insts, nslots, live);
     return hipGetLastError();
 }
 

@@ -72,6 +72,7 @@ hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint3
 hipError_t launch_apply(const l2o_apply_item *items, uint32_t nitems, l2o_inst *insts, unsigned *live,
                         hipStream_t stream);
 hipError_t launch_ils(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStream_t stream);
+hipError_t launch_group(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStream_t stream);
 hipError_t launch_length(const l2o_inst *insts, uint32_t slot, uint32_t n, unsigned long long *out,
                          hipStream_t stream);
 hipError_t launch_sqrt_ulp(uint32_t lo, uint32_t hi, unsigned *max_ulp, hipStream_t stream);

@@ -30,7 +30,11 @@ enum l2o_phase : uint32_t {
     L2O_LO = 1,       // local search running: sweep + apply every step
     L2O_LO_DONE = 2,  // local optimum reached, ILS bookkeeping pending
     L2O_DONE = 3,     // finished (result ready)
+    L2O_WAIT = 4,     // chain of a group at a local optimum, waiting for the others
+    L2O_KICK = 5,     // chain of a group: kick from `kick_src`'s best, then local search
 };
+
+#define L2O_NO_GROUP 0xFFFFFFFFu
 
 struct __attribute__((aligned(16))) l2o_pt {
     int32_t x, y;  // coordinates, 0 <= x, y < 2^22
@@ -59,6 +63,10 @@ struct l2o_inst {
     uint32_t kick;
     uint32_t have_best;
     uint32_t metric;      // edge length: L2O_EUC_2D, L2O_CEIL_2D or L2O_ATT
+    uint32_t group_lead;  // multi-start ILS: first slot of the group (L2O_NO_GROUP: none)
+    uint32_t group_size;  // chains in the group (slots group_lead .. group_lead+size-1)
+    uint32_t improved;    // the chain's last local search improved its own best
+    uint32_t kick_src;    // slot whose best tour the next kick starts from
     float est_scale;      // FP32 tier: D ~ sqrt(s) * est_scale + est_add
     float est_add;
     // ---- dynamic

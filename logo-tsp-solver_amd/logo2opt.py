@@ -33,7 +33,8 @@ EXPORTS = [
     "logo2opt_random_perm", "logo2opt_nn_tour", "logo2opt_xy_to_coords", "logo2opt_sqrt_ulp_check",
     "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
     "logo2opt_engine_time_sweep", "logo2opt_create_metric", "logo2opt_tsp_read", "logo2opt_tour_read",
-    "logo2opt_tour_write", "logo2opt_host_route_length", "logo2opt_mf_tour",
+    "logo2opt_tour_write", "logo2opt_host_route_length", "logo2opt_mf_tour", "logo2opt_ils_multistart",
+    "logo2opt_engine_add_group", "logo2opt_engine_group_result",
 ]
 
 
@@ -106,6 +107,11 @@ def load():
                                     C.c_uint32]
     L.logo2opt_ils.argtypes = [vp, u32p, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), vp]
     L.logo2opt_route_length.argtypes = [vp, u32p, C.POINTER(C.c_uint64)]
+    L.logo2opt_ils_multistart.argtypes = [vp, u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int,
+                                          C.POINTER(C.c_uint64), vp]
+    L.logo2opt_engine_add_group.argtypes = [vp, C.c_uint32, vp, u32p, C.POINTER(Job), C.c_uint32,
+                                            C.POINTER(C.c_int32)]
+    L.logo2opt_engine_group_result.argtypes = [vp, C.c_int32, vp, C.POINTER(Result), C.POINTER(C.c_int32)]
     L.logo2opt_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.logo2opt_set_timing.argtypes = [vp, C.c_int]
     L.logo2opt_engine_set_timing.argtypes = [vp, C.c_int]
@@ -341,6 +347,14 @@ class Context:
                "logo2opt_ils")
         return r, int(best.value), rl
 
+    def ils_multistart(self, route, seed, rounds, chains, kick=KICK_DOUBLE_BRIDGE):
+        r = np.array(route, np.uint32)
+        best = C.c_uint64()
+        rl = np.zeros(rounds + 1, np.uint64)
+        _check(load().logo2opt_ils_multistart(self.h, r, seed, rounds, chains, kick, C.byref(best), rl.ctypes.data),
+               "logo2opt_ils_multistart")
+        return r, int(best.value), rl
+
     def route_length(self, route):
         L = C.c_uint64()
         _check(load().logo2opt_route_length(self.h, np.ascontiguousarray(route, np.uint32), C.byref(L)),
@@ -403,6 +417,28 @@ class Engine:
             return None
         _check(rc, "logo2opt_engine_add")
         return s.value
+
+    def add_group(self, coords, route, chains, ils_rounds, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D):
+        """Multi-start ILS group; returns its first slot, or None when full."""
+        route = np.ascontiguousarray(route, np.uint32)
+        j = Job(ils_rounds, kick, seed, 0, metric)
+        s = C.c_int32()
+        rc = load().logo2opt_engine_add_group(self.h, len(route), C.cast(coords, vp), route, C.byref(j), chains,
+                                              C.byref(s))
+        if rc == -6:
+            return None
+        _check(rc, "logo2opt_engine_add_group")
+        return s.value
+
+    def group_result(self, first_slot, n):
+        res = Result()
+        r = np.zeros(n, np.uint32)
+        b = C.c_int32()
+        _check(load().logo2opt_engine_group_result(self.h, first_slot, r.ctypes.data, C.byref(res), C.byref(b)),
+               "logo2opt_engine_group_result")
+        out = {f: getattr(res, f) for f, _ in Result._fields_}
+        out.update(route=r, best_chain=b.value)
+        return out
 
     def run(self, max_steps=0xFFFFFFFF, sync_every=32):
         live = C.c_uint32()

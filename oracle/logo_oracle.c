@@ -591,3 +591,72 @@ uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
     return orc_ils_m(n, xy, route, seed, rounds, kick_kind, threads, ORC_METRIC_EUC_2D,
                      round_lengths, total_moves);
 }
+
+/* ILSGlobalSolverMT::optimize with comm = 1 (ILSGlobalSolverMT.cpp:86-195)
+ * restated deterministically: `chains` chains start from route; round 0 is a
+ * local search of every chain; in every later round each chain kicks (its own
+ * generator, seed + c) its own best if its last local search improved that
+ * best, else the group best, and runs a local search; a chain keeps a result
+ * iff it is strictly shorter than its best.  After each round the group best
+ * is the chain with the smallest best (lowest chain on ties).  The reference
+ * shares the global best through a mutex between free-running threads, so
+ * its sequence depends on timing; here rounds are synchronised.  route
+ * receives the group best; round_lengths (rounds+1) the group best per round.
+ * Returns the group best length. */
+uint64_t orc_ils_multistart(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                            uint32_t rounds, uint32_t chains, int kick_kind, int threads,
+                            int metric, uint64_t *round_lengths, uint64_t *total_moves)
+{
+    uint32_t **best = (uint32_t **)malloc(sizeof(uint32_t *) * chains);
+    uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *gbest = (uint32_t *)malloc(sizeof(uint32_t) * n); /* group best at the last barrier */
+    uint64_t *bl = (uint64_t *)malloc(sizeof(uint64_t) * chains);
+    int *improved = (int *)malloc(sizeof(int) * chains);
+    orc_rng_t *rng = (orc_rng_t *)malloc(sizeof(orc_rng_t) * chains);
+    uint64_t tm = 0, L;
+    uint32_t g = 0;
+    for (uint32_t c = 0; c < chains; c++) {
+        best[c] = (uint32_t *)malloc(sizeof(uint32_t) * n);
+        orc_rng_seed(&rng[c], seed + c);
+        memcpy(cur, route, sizeof(uint32_t) * n);
+        tm += orc_local_search_m(n, xy, cur, threads, metric, 0, NULL, 0, NULL, &L);
+        memcpy(best[c], cur, sizeof(uint32_t) * n);
+        bl[c] = L;
+        improved[c] = 1;
+        if (bl[c] < bl[g]) g = c;
+    }
+    if (round_lengths) round_lengths[0] = bl[g];
+    for (uint32_t r = 0; r < rounds; r++) {
+        /* every chain of this round sees the group best of the previous one,
+         * even after the chain holding it has moved on within the round */
+        memcpy(gbest, best[g], sizeof(uint32_t) * n);
+        for (uint32_t c = 0; c < chains; c++) {
+            memcpy(cur, improved[c] ? best[c] : gbest, sizeof(uint32_t) * n);
+            if (kick_kind == ORC_KICK_REFERENCE)
+                orc_ref_kick(n, cur, &rng[c]);
+            else
+                orc_double_bridge(n, cur, &rng[c], NULL);
+            tm += orc_local_search_m(n, xy, cur, threads, metric, 0, NULL, 0, NULL, &L);
+            improved[c] = L < bl[c];
+            if (improved[c]) {
+                bl[c] = L;
+                memcpy(best[c], cur, sizeof(uint32_t) * n);
+            }
+        }
+        g = 0;
+        for (uint32_t c = 1; c < chains; c++)
+            if (bl[c] < bl[g]) g = c;
+        if (round_lengths) round_lengths[r + 1] = bl[g];
+    }
+    memcpy(route, best[g], sizeof(uint32_t) * n);
+    L = bl[g];
+    for (uint32_t c = 0; c < chains; c++) free(best[c]);
+    free(best);
+    free(cur);
+    free(gbest);
+    free(bl);
+    free(improved);
+    free(rng);
+    if (total_moves) *total_moves = tm;
+    return L;
+}

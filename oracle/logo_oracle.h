@@ -136,6 +136,12 @@ uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed
                    uint32_t rounds, int kick_kind, int threads, int metric,
                    uint64_t *round_lengths, uint64_t *total_moves);
 
+/* Multi-start ILS with a shared best, rounds synchronised across chains
+ * (ILSGlobalSolverMT.cpp:86-195 with comm = 1); chain c seeds seed + c. */
+uint64_t orc_ils_multistart(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                            uint32_t rounds, uint32_t chains, int kick_kind, int threads,
+                            int metric, uint64_t *round_lengths, uint64_t *total_moves);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,9 @@ def _load_oracle():
     L.orc_ils_m.argtypes = [C.c_uint32, i32p, u32p, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int,
                             C.c_void_p, C.POINTER(C.c_uint64)]
     L.orc_ils_m.restype = C.c_uint64
+    L.orc_ils_multistart.argtypes = [C.c_uint32, i32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                     C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]
+    L.orc_ils_multistart.restype = C.c_uint64
     return L
 
 
@@ -160,6 +163,15 @@ def ils(xy, route, seed, rounds, kick="double_bridge", threads=1, metric=METRIC_
     tm = C.c_uint64()
     best = lib.orc_ils_m(len(r), xy, r, seed, rounds, 0 if kick == "double_bridge" else 1, threads, metric,
                          rl.ctypes.data, C.byref(tm))
+    return r, int(best), rl, int(tm.value)
+
+
+def ils_multistart(xy, route, seed, rounds, chains, kick="double_bridge", threads=1, metric=METRIC_EUC_2D):
+    r = np.array(route, np.uint32)
+    rl = np.zeros(rounds + 1, np.uint64)
+    tm = C.c_uint64()
+    best = lib.orc_ils_multistart(len(r), xy, r, seed, rounds, chains, 0 if kick == "double_bridge" else 1,
+                                  threads, metric, rl.ctypes.data, C.byref(tm))
     return r, int(best), rl, int(tm.value)
 
 

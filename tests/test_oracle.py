@@ -417,3 +417,48 @@ def test_reference_kick_matches_rotate_reverse():
         got = r.copy()
         R.ref_kick(got, n, seed)
         assert got.tolist() == O.rotate_reverse(r, a, b).tolist()
+
+
+def test_oracle_multistart_ils_properties():
+    # one chain is plain ILS; group best per round never increases and is no
+    # worse than any single chain started from the same seed
+    xy = O.gen_uniform(150, 10**4, 21)
+    r0 = O.random_perm(150, 22)
+    r1, L1, rl1, _ = O.ils(xy, r0, 5, 8)
+    rm, Lm, rlm, _ = O.ils_multistart(xy, r0, 5, 8, 1)
+    assert (Lm, rm.tolist(), rlm.tolist()) == (L1, r1.tolist(), rl1.tolist())
+    rg, Lg, rlg, _ = O.ils_multistart(xy, r0, 5, 8, 4)
+    assert np.all(np.diff(rlg.astype(np.int64)) <= 0)
+    assert Lg == O.route_length(xy, rg) == int(rlg[-1])
+    # the group best after round 0 is the common first local optimum
+    assert int(rlg[0]) == int(rl1[0])
+
+
+def test_oracle_multistart_against_round_loop():
+    # the same rule written as an explicit round loop over the pinned local
+    # search and double-bridge: at each barrier the group best tour is fixed,
+    # then every chain kicks (its own best if it improved, else that tour).
+    # With this instance, reading the group best's tour after its chain has
+    # moved on within a round gives a different history.
+    n, chains, rounds, seed = 90, 4, 8, 17
+    xy = O.gen_uniform(n, 3000, 5)
+    r0 = O.random_perm(n, 6)
+    gens = [O.rng(seed + c) for c in range(chains)]
+    lo = O.local_search(xy, r0)
+    best = [lo[0].copy() for _ in range(chains)]
+    bl = [lo[1]] * chains
+    imp = [True] * chains
+    g = 0
+    hist = [bl[0]]
+    for _ in range(rounds):
+        snap = best[g].copy()
+        for c in range(chains):
+            kicked, _ = O.double_bridge(best[c] if imp[c] else snap, gens[c])
+            r, L = O.local_search(xy, kicked)[:2]
+            imp[c] = L < bl[c]
+            if imp[c]:
+                best[c], bl[c] = r, L
+        g = min(range(chains), key=lambda c: (bl[c], c))
+        hist.append(bl[g])
+    rg, Lg, rlg, _ = O.ils_multistart(xy, r0, seed, rounds, chains)
+    assert Lg == bl[g] and rg.tolist() == best[g].tolist() and rlg.tolist() == hist
