@@ -71,10 +71,32 @@ def config2(seed=2):
     return n, xy, r0
 
 
+def exact_simd_port(n, xy, r0, budget_s):
+    """The strongest CPU sweep here: the oracle's exact SIMD sweep
+    (orc_step_fast: integer L1 first test over 16 or 8 columns, exact lengths
+    for the survivors) on up to 16 threads, on the same tour."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    xy = np.ascontiguousarray(xy, np.int32)
+    sweeps, t0 = 0, time.perf_counter()
+    while True:
+        O.step_fast(xy, r0, threads)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or sweeps >= 400:
+            break
+    return {"value": sweeps * pairs(n) / el, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{sweeps} exact SIMD sweeps ({O.fast_isa()}, {el:.1f} s, {threads} threads) of the "
+                      "33,810-city clustered instance at its NN start tour; same move as the oracle"}
+
+
 def cpu_baseline(n, xy, r0, budget_s):
     """Bounded samples of the reference's CPU sweep on this host: its pthreads
     CPUMTSolver::optimizeStep on up to 16 threads (the CPU share of one GPU
-    here) as the baseline, and its single-thread CPUSolver::optimizeStep."""
+    here) as the baseline, and its single-thread CPUSolver::optimizeStep; the
+    exact SIMD port beside them."""
     ref_avx = os.path.join(ROOT, "oracle", "_ref", "libref2opt_avx.so")
     ref_sc = os.path.join(ROOT, "oracle", "_ref", "libref2opt.so")
     path = ref_avx if os.path.exists(ref_avx) else ref_sc if os.path.exists(ref_sc) else None
@@ -96,18 +118,18 @@ def cpu_baseline(n, xy, r0, budget_s):
                     return sweeps, el
 
         variant = "AVX build (USE_CPU_SIMD, USE_AVX)" if path == ref_avx else "scalar build"
-        s1, e1 = sample(lambda: R.ref_step(fxy, r0, n, out), 30, budget_s / 2)
+        s1, e1 = sample(lambda: R.ref_step(fxy, r0, n, out), 30, budget_s / 3)
         single = {"value": s1 * pairs(n) / e1, "cores": 1,
                   "sample": f"{s1} CPUSolver::optimizeStep sweeps ({e1:.1f} s), one thread as --mode=cpu"}
         if hasattr(R, "ref_step_mt"):
             R.ref_step_mt.argtypes = [f32, u32, C.c_uint32, C.c_int32, i32]
             threads = min(16, len(os.sched_getaffinity(0)))
-            sm, em = sample(lambda: R.ref_step_mt(fxy, r0, n, threads, out), 200, budget_s / 2)
+            sm, em = sample(lambda: R.ref_step_mt(fxy, r0, n, threads, out), 200, budget_s / 3)
             return {"value": sm * pairs(n) / em, "unit": "evals/s", "cores": threads, "kind": "reference",
                     "sample": f"{sm} CPUMTSolver::optimizeStep sweeps ({em:.1f} s, {threads} pthreads as "
                               f"--cores={threads}) of the 33,810-city clustered instance at its NN start tour, "
                               f"reference sources {variant}",
-                    "single_thread": single}
+                    "single_thread": single, "exact_simd_port": exact_simd_port(n, xy, r0, budget_s / 3)}
         single.update({"unit": "evals/s", "kind": "reference",
                        "sample": single["sample"] + f" of the 33,810-city instance, reference sources {variant}"})
         return single

@@ -660,3 +660,341 @@ uint64_t orc_ils_multistart(uint32_t n, const int32_t *xy, uint32_t *route, uint
     if (total_moves) *total_moves = tm;
     return L;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Exact SIMD sweep: the CPU baseline (same result as orc_step_m)             */
+/* ------------------------------------------------------------------------ */
+/* The cpuSolver.cpp:148-197 / cpuMTSolver.cpp:79-340 analogue done exactly:
+ * a candidate (i, J) can only beat the running best `best` if
+ *   L1(i-1, J-1) + L1(i, J) - W[J] <= ceil(sqrt(g) best) + W[i],
+ * with W[k] = ceil(sqrt(g) D[k]) and g = 2 (EUC_2D, CEIL_2D) or 20 (ATT):
+ * |dx| + |dy| <= sqrt(2) d and d < D + 1/2, d <= D, d <= sqrt(10) D for the
+ * three metrics.  The integer test runs 16 (AVX-512, four rows per block of
+ * column loads) or 8 (AVX2) columns at a time; the rare survivors get the
+ * exact lengths and the minimum of the key (gain, i, J), so the result is
+ * the lexicographic minimum like orc_step_m.  Threads take i-ranges of equal
+ * pair counts and merge in i order. */
+#include <immintrin.h>
+
+static int32_t ceil_sqrtg_i(int32_t d, int g)
+{
+    if (d <= 0) return 0;
+    uint64_t s = (uint64_t)g * (uint64_t)d * (uint64_t)d;
+    return (int32_t)(isqrt_u64(s) + 1); /* g d^2 is never a square for d >= 1 */
+}
+
+/* ceil(sqrt(g) * (best_gain_allowed + 1)) for the largest gain still wanted:
+ * improving moves only (best = 0) -> 0; else best - 1 + 1 = best <= -1 */
+static int32_t tier1_threshold(int32_t best, int g)
+{
+    return best == 0 ? 0 : 1 - ceil_sqrtg_i(-best, g);
+}
+
+typedef struct {
+    uint32_t n, i_lo, i_hi;
+    int metric, g;
+    const int32_t *x, *y, *D, *W;
+    int32_t best;
+    uint32_t bi, bj;
+    uint64_t survivors;
+    int isa;
+} fast_job;
+
+/* exact check of candidate column b of row a (i = a+1, J = b+1).  Rows are
+ * scanned in blocks, so candidates do not arrive in (i, J) order: the best is
+ * the minimum of the full key (gain, i, J), and the threshold admits gains
+ * equal to the best (tb = ceil(sqrt(g) (best + 1))). */
+static inline void fast_exact(fast_job *jb, uint32_t a, uint32_t b, int32_t *tb)
+{
+    const int32_t *x = jb->x, *y = jb->y;
+    int32_t g = (int32_t)dist_xy(x[a], y[a], x[b], y[b], jb->metric) +
+                (int32_t)dist_xy(x[a + 1], y[a + 1], x[b + 1], y[b + 1], jb->metric) - jb->D[a + 1] -
+                jb->D[b + 1];
+    jb->survivors++;
+    if (g < 0 && (g < jb->best || (g == jb->best && (a + 1 < jb->bi || (a + 1 == jb->bi && b + 1 < jb->bj))))) {
+        jb->best = g;
+        jb->bi = a + 1;
+        jb->bj = b + 1;
+        *tb = tier1_threshold(g + 1, jb->g);
+    }
+}
+
+static inline int32_t fast_l1(const fast_job *jb, uint32_t a, uint32_t b)
+{
+    int32_t dx = jb->x[a] - jb->x[b], dy = jb->y[a] - jb->y[b];
+    return (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy);
+}
+
+static int32_t fast_tb(const fast_job *jb)
+{
+    return jb->best == 0 ? 0 : tier1_threshold(jb->best + 1, jb->g);
+}
+
+static void fast_rows_scalar(fast_job *jb)
+{
+    int32_t tb = fast_tb(jb);
+    for (uint32_t a = jb->i_lo - 1; a + 1 < jb->i_hi; a++)
+        for (uint32_t b = a + 2; b + 2 <= jb->n; b++)
+            if (fast_l1(jb, a, b) + fast_l1(jb, a + 1, b + 1) - jb->W[b + 1] <= tb + jb->W[a + 1])
+                fast_exact(jb, a, b, &tb);
+}
+
+#define FAST_L1_SUM8(r)                                                                                    \
+    _mm256_sub_epi32(                                                                                      \
+        _mm256_add_epi32(_mm256_add_epi32(_mm256_abs_epi32(_mm256_sub_epi32(cx0, px##r)),                   \
+                                          _mm256_abs_epi32(_mm256_sub_epi32(cy0, py##r))),                  \
+                         _mm256_add_epi32(_mm256_abs_epi32(_mm256_sub_epi32(cx1, px##r##n)),                \
+                                          _mm256_abs_epi32(_mm256_sub_epi32(cy1, py##r##n)))),              \
+        w1)
+/* s <= t as an 8-bit mask */
+#define FAST_LE8(s_, t_) ((unsigned)_mm256_movemask_ps(_mm256_castsi256_ps(_mm256_cmpgt_epi32((t_), (s_)))))
+
+__attribute__((target("avx2"))) static void fast_rows_avx2(fast_job *jb)
+{
+    const int32_t *x = jb->x, *y = jb->y, *W = jb->W;
+    int32_t tb = fast_tb(jb);
+    const uint32_t end = jb->n - 1; /* columns b <= n-2 */
+    uint32_t a0 = jb->i_lo - 1;
+    const uint32_t a_end = jb->i_hi - 1;
+    for (; a0 + 4 <= a_end; a0 += 4) {
+        const uint32_t bv = a0 + 5;
+        for (uint32_t r = 0; r < 4; r++)
+            for (uint32_t b = a0 + r + 2; b < bv && b < end; b++)
+                if (fast_l1(jb, a0 + r, b) + fast_l1(jb, a0 + r + 1, b + 1) - W[b + 1] <= tb + W[a0 + r + 1])
+                    fast_exact(jb, a0 + r, b, &tb);
+        const __m256i px0 = _mm256_set1_epi32(x[a0]), py0 = _mm256_set1_epi32(y[a0]);
+        const __m256i px1 = _mm256_set1_epi32(x[a0 + 1]), py1 = _mm256_set1_epi32(y[a0 + 1]);
+        const __m256i px2 = _mm256_set1_epi32(x[a0 + 2]), py2 = _mm256_set1_epi32(y[a0 + 2]);
+        const __m256i px3 = _mm256_set1_epi32(x[a0 + 3]), py3 = _mm256_set1_epi32(y[a0 + 3]);
+        const __m256i px4 = _mm256_set1_epi32(x[a0 + 4]), py4 = _mm256_set1_epi32(y[a0 + 4]);
+        const __m256i px0n = px1, py0n = py1, px1n = px2, py1n = py2, px2n = px3, py2n = py3, px3n = px4,
+                      py3n = py4;
+        /* s <= tb + W  <=>  tb + W + 1 > s */
+        __m256i t0 = _mm256_set1_epi32(tb + W[a0 + 1] + 1), t1 = _mm256_set1_epi32(tb + W[a0 + 2] + 1);
+        __m256i t2 = _mm256_set1_epi32(tb + W[a0 + 3] + 1), t3 = _mm256_set1_epi32(tb + W[a0 + 4] + 1);
+        uint32_t b = bv;
+        for (; b + 8 <= end; b += 8) {
+            const __m256i cx0 = _mm256_loadu_si256((const __m256i *)(x + b));
+            const __m256i cy0 = _mm256_loadu_si256((const __m256i *)(y + b));
+            const __m256i cx1 = _mm256_loadu_si256((const __m256i *)(x + b + 1));
+            const __m256i cy1 = _mm256_loadu_si256((const __m256i *)(y + b + 1));
+            const __m256i w1 = _mm256_loadu_si256((const __m256i *)(W + b + 1));
+            const unsigned m0 = FAST_LE8(FAST_L1_SUM8(0), t0), m1 = FAST_LE8(FAST_L1_SUM8(1), t1);
+            const unsigned m2 = FAST_LE8(FAST_L1_SUM8(2), t2), m3 = FAST_LE8(FAST_L1_SUM8(3), t3);
+            if (__builtin_expect((m0 | m1 | m2 | m3) != 0, 0)) {
+                const unsigned mm[4] = {m0, m1, m2, m3};
+                for (uint32_t r = 0; r < 4; r++)
+                    for (unsigned m = mm[r]; m; m &= m - 1) fast_exact(jb, a0 + r, b + (unsigned)__builtin_ctz(m), &tb);
+                t0 = _mm256_set1_epi32(tb + W[a0 + 1] + 1);
+                t1 = _mm256_set1_epi32(tb + W[a0 + 2] + 1);
+                t2 = _mm256_set1_epi32(tb + W[a0 + 3] + 1);
+                t3 = _mm256_set1_epi32(tb + W[a0 + 4] + 1);
+            }
+        }
+        for (uint32_t r = 0; r < 4; r++)
+            for (uint32_t c = b; c < end; c++)
+                if (fast_l1(jb, a0 + r, c) + fast_l1(jb, a0 + r + 1, c + 1) - W[c + 1] <= tb + W[a0 + r + 1])
+                    fast_exact(jb, a0 + r, c, &tb);
+    }
+    for (; a0 < a_end; a0++)
+        for (uint32_t b = a0 + 2; b < end; b++)
+            if (fast_l1(jb, a0, b) + fast_l1(jb, a0 + 1, b + 1) - W[b + 1] <= tb + W[a0 + 1]) fast_exact(jb, a0, b, &tb);
+}
+
+/* four rows per block of column loads (AVX-512); leftover rows go one by one */
+#define FAST_L1_SUM(r)                                                                                     \
+    _mm512_sub_epi32(                                                                                      \
+        _mm512_add_epi32(_mm512_add_epi32(_mm512_abs_epi32(_mm512_sub_epi32(cx0, px##r)),                   \
+                                          _mm512_abs_epi32(_mm512_sub_epi32(cy0, py##r))),                  \
+                         _mm512_add_epi32(_mm512_abs_epi32(_mm512_sub_epi32(cx1, px##r##n)),                \
+                                          _mm512_abs_epi32(_mm512_sub_epi32(cy1, py##r##n)))),              \
+        w1)
+
+__attribute__((target("avx512f"))) static void fast_row1_avx512(fast_job *jb, uint32_t a, int32_t *tb)
+{
+    const int32_t *x = jb->x, *y = jb->y, *W = jb->W;
+    const uint32_t end = jb->n - 1; /* columns b <= n-2 */
+    const __m512i px0 = _mm512_set1_epi32(x[a]), py0 = _mm512_set1_epi32(y[a]);
+    const __m512i px0n = _mm512_set1_epi32(x[a + 1]), py0n = _mm512_set1_epi32(y[a + 1]);
+    uint32_t b = a + 2;
+    for (; b + 16 <= end; b += 16) {
+        const __m512i cx0 = _mm512_loadu_si512(x + b), cy0 = _mm512_loadu_si512(y + b);
+        const __m512i cx1 = _mm512_loadu_si512(x + b + 1), cy1 = _mm512_loadu_si512(y + b + 1);
+        const __m512i w1 = _mm512_loadu_si512(W + b + 1);
+        unsigned m = (unsigned)_mm512_cmple_epi32_mask(FAST_L1_SUM(0), _mm512_set1_epi32(*tb + W[a + 1]));
+        while (m) {
+            unsigned k = (unsigned)__builtin_ctz(m);
+            m &= m - 1;
+            fast_exact(jb, a, b + k, tb);
+        }
+    }
+    for (; b < end; b++)
+        if (fast_l1(jb, a, b) + fast_l1(jb, a + 1, b + 1) - W[b + 1] <= *tb + W[a + 1]) fast_exact(jb, a, b, tb);
+}
+
+__attribute__((target("avx512f"))) static void fast_rows_avx512(fast_job *jb)
+{
+    const int32_t *x = jb->x, *y = jb->y, *W = jb->W;
+    int32_t tb = fast_tb(jb);
+    const uint32_t end = jb->n - 1;
+    uint32_t a0 = jb->i_lo - 1;
+    const uint32_t a_end = jb->i_hi - 1; /* rows a < a_end */
+    for (; a0 + 4 <= a_end; a0 += 4) {
+        /* columns below a0 + 5 belong to only some of the four rows: scalar */
+        const uint32_t bv = a0 + 5;
+        for (uint32_t r = 0; r < 4; r++)
+            for (uint32_t b = a0 + r + 2; b < bv && b < end; b++)
+                if (fast_l1(jb, a0 + r, b) + fast_l1(jb, a0 + r + 1, b + 1) - W[b + 1] <= tb + W[a0 + r + 1])
+                    fast_exact(jb, a0 + r, b, &tb);
+        const __m512i px0 = _mm512_set1_epi32(x[a0]), py0 = _mm512_set1_epi32(y[a0]);
+        const __m512i px1 = _mm512_set1_epi32(x[a0 + 1]), py1 = _mm512_set1_epi32(y[a0 + 1]);
+        const __m512i px2 = _mm512_set1_epi32(x[a0 + 2]), py2 = _mm512_set1_epi32(y[a0 + 2]);
+        const __m512i px3 = _mm512_set1_epi32(x[a0 + 3]), py3 = _mm512_set1_epi32(y[a0 + 3]);
+        const __m512i px4 = _mm512_set1_epi32(x[a0 + 4]), py4 = _mm512_set1_epi32(y[a0 + 4]);
+        const __m512i px0n = px1, py0n = py1, px1n = px2, py1n = py2, px2n = px3, py2n = py3, px3n = px4,
+                      py3n = py4;
+        __m512i t0 = _mm512_set1_epi32(tb + W[a0 + 1]), t1 = _mm512_set1_epi32(tb + W[a0 + 2]);
+        __m512i t2 = _mm512_set1_epi32(tb + W[a0 + 3]), t3 = _mm512_set1_epi32(tb + W[a0 + 4]);
+        uint32_t b = bv;
+        for (; b + 16 <= end; b += 16) {
+            const __m512i cx0 = _mm512_loadu_si512(x + b), cy0 = _mm512_loadu_si512(y + b);
+            const __m512i cx1 = _mm512_loadu_si512(x + b + 1), cy1 = _mm512_loadu_si512(y + b + 1);
+            const __m512i w1 = _mm512_loadu_si512(W + b + 1);
+            const __mmask16 m0 = _mm512_cmple_epi32_mask(FAST_L1_SUM(0), t0);
+            const __mmask16 m1 = _mm512_cmple_epi32_mask(FAST_L1_SUM(1), t1);
+            const __mmask16 m2 = _mm512_cmple_epi32_mask(FAST_L1_SUM(2), t2);
+            const __mmask16 m3 = _mm512_cmple_epi32_mask(FAST_L1_SUM(3), t3);
+            if (__builtin_expect((m0 | m1 | m2 | m3) != 0, 0)) {
+                const unsigned mm[4] = {m0, m1, m2, m3};
+                for (uint32_t r = 0; r < 4; r++)
+                    for (unsigned m = mm[r]; m; m &= m - 1) fast_exact(jb, a0 + r, b + (unsigned)__builtin_ctz(m), &tb);
+                t0 = _mm512_set1_epi32(tb + W[a0 + 1]);
+                t1 = _mm512_set1_epi32(tb + W[a0 + 2]);
+                t2 = _mm512_set1_epi32(tb + W[a0 + 3]);
+                t3 = _mm512_set1_epi32(tb + W[a0 + 4]);
+            }
+        }
+        for (uint32_t r = 0; r < 4; r++)
+            for (uint32_t c = b; c < end; c++)
+                if (fast_l1(jb, a0 + r, c) + fast_l1(jb, a0 + r + 1, c + 1) - W[c + 1] <= tb + W[a0 + r + 1])
+                    fast_exact(jb, a0 + r, c, &tb);
+    }
+    for (; a0 < a_end; a0++) fast_row1_avx512(jb, a0, &tb);
+}
+
+/* the widest variant this CPU runs; ORC_FAST_ISA=avx2|scalar narrows it
+ * (used by the tests to cover every variant on one machine) */
+static int fast_isa_id(void)
+{
+    const char *want = getenv("ORC_FAST_ISA");
+    __builtin_cpu_init();
+    if (want && !strcmp(want, "scalar")) return 0;
+    if (want && !strcmp(want, "avx2")) return __builtin_cpu_supports("avx2") ? 1 : 0;
+    if (__builtin_cpu_supports("avx512f")) return 2;
+    if (__builtin_cpu_supports("avx2")) return 1;
+    return 0;
+}
+
+const char *orc_fast_isa(void)
+{
+    static const char *names[] = {"scalar", "avx2", "avx512"};
+    return names[fast_isa_id()];
+}
+
+static void *fast_thread(void *arg)
+{
+    fast_job *jb = (fast_job *)arg;
+    if (jb->isa == 2) fast_rows_avx512(jb);
+    else if (jb->isa == 1) fast_rows_avx2(jb);
+    else fast_rows_scalar(jb);
+    return NULL;
+}
+
+int orc_step_fast(uint32_t n, const int32_t *xy, const uint32_t *route, int threads, int metric,
+                  orc_best2_out *out, uint64_t *survivors)
+{
+    out->i = out->j = 0;
+    out->minchange = 0;
+    if (survivors) *survivors = 0;
+    if (n < 4) return 0;
+    int32_t *x = (int32_t *)malloc(sizeof(int32_t) * (n + 16));
+    int32_t *y = (int32_t *)malloc(sizeof(int32_t) * (n + 16));
+    int32_t *D = (int32_t *)malloc(sizeof(int32_t) * (n + 16));
+    int32_t *W = (int32_t *)malloc(sizeof(int32_t) * (n + 16));
+    const int g = metric == ORC_METRIC_ATT ? 20 : 2;
+    for (uint32_t k = 0; k < n; k++) {
+        x[k] = xy[2 * route[k]];
+        y[k] = xy[2 * route[k] + 1];
+    }
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t p = k == 0 ? n - 1 : k - 1;
+        D[k] = (int32_t)dist_xy(x[p], y[p], x[k], y[k], metric);
+        W[k] = ceil_sqrtg_i(D[k], g);
+    }
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((uint32_t)threads > n - 3) threads = (int)(n - 3);
+    fast_job jobs[256];
+    pthread_t tid[256];
+    double total = (double)(n - 2) * (double)(n - 3) / 2.0;
+    uint32_t lo = 1;
+    const int isa = fast_isa_id();
+    for (int t = 0; t < threads; t++) {
+        uint32_t hi;
+        if (t == threads - 1) {
+            hi = n - 2;
+        } else {
+            double target = total * (double)(t + 1) / (double)threads;
+            hi = lo;
+            while (hi < n - 2) {
+                double h = (double)hi;
+                if ((h - 1.0) * (double)(n - 2) - (h - 1.0) * h / 2.0 >= target) break;
+                hi++;
+            }
+        }
+        fast_job *jb = &jobs[t];
+        jb->isa = isa;
+        jb->n = n;
+        jb->i_lo = lo;
+        jb->i_hi = hi;
+        jb->metric = metric;
+        jb->g = g;
+        jb->x = x;
+        jb->y = y;
+        jb->D = D;
+        jb->W = W;
+        jb->best = 0;
+        jb->bi = jb->bj = 0;
+        jb->survivors = 0;
+        lo = hi;
+    }
+    if (threads == 1) {
+        fast_thread(&jobs[0]);
+    } else {
+        for (int t = 0; t < threads; t++) pthread_create(&tid[t], NULL, fast_thread, &jobs[t]);
+        for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    }
+    int32_t best = 0;
+    uint32_t bi = 0, bj = 0;
+    uint64_t sv = 0;
+    for (int t = 0; t < threads; t++) {
+        sv += jobs[t].survivors;
+        if (jobs[t].i_lo < jobs[t].i_hi && jobs[t].best < best) {
+            best = jobs[t].best;
+            bi = jobs[t].bi;
+            bj = jobs[t].bj;
+        }
+    }
+    free(x);
+    free(y);
+    free(D);
+    free(W);
+    if (survivors) *survivors = sv;
+    if (best < 0) {
+        out->i = bi;
+        out->j = bj;
+        out->minchange = best;
+        return 1;
+    }
+    return 0;
+}

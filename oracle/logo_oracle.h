@@ -142,6 +142,14 @@ uint64_t orc_ils_multistart(uint32_t n, const int32_t *xy, uint32_t *route, uint
                             uint32_t rounds, uint32_t chains, int kick_kind, int threads,
                             int metric, uint64_t *round_lengths, uint64_t *total_moves);
 
+/* Exact SIMD sweep used as the strongest CPU baseline: the same result as
+ * orc_step_m, with an integer L1 first test over 16 (AVX-512) or 8 (AVX2)
+ * columns at a time; survivors (optional) receives how many candidates
+ * reached the exact lengths.  orc_fast_isa names the variant in use. */
+int orc_step_fast(uint32_t n, const int32_t *xy, const uint32_t *route, int threads, int metric,
+                  orc_best2_out *out, uint64_t *survivors);
+const char *orc_fast_isa(void);
+
 #ifdef __cplusplus
 }
 #endif

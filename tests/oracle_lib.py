@@ -71,6 +71,9 @@ def _load_oracle():
     L.orc_ils_multistart.argtypes = [C.c_uint32, i32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
                                      C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]
     L.orc_ils_multistart.restype = C.c_uint64
+    L.orc_step_fast.argtypes = [C.c_uint32, i32p, u32p, C.c_int, C.c_int, C.POINTER(Best2Out),
+                                C.POINTER(C.c_uint64)]
+    L.orc_fast_isa.restype = C.c_char_p
     return L
 
 
@@ -138,6 +141,19 @@ def step(xy, route, threads=1, metric=METRIC_EUC_2D):
     o = Best2Out()
     ok = lib.orc_step_m(len(route), xy, np.ascontiguousarray(route, np.uint32), threads, metric, C.byref(o))
     return bool(ok), (o.i, o.j, o.minchange)
+
+
+def step_fast(xy, route, threads=1, metric=METRIC_EUC_2D):
+    """Exact SIMD sweep (the CPU baseline): same result as step()."""
+    o = Best2Out()
+    sv = C.c_uint64()
+    ok = lib.orc_step_fast(len(route), xy, np.ascontiguousarray(route, np.uint32), threads, metric, C.byref(o),
+                           C.byref(sv))
+    return bool(ok), (o.i, o.j, o.minchange)
+
+
+def fast_isa():
+    return lib.orc_fast_isa().decode()
 
 
 def apply(route, i, j):

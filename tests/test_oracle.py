@@ -462,3 +462,18 @@ def test_oracle_multistart_against_round_loop():
         hist.append(bl[g])
     rg, Lg, rlg, _ = O.ils_multistart(xy, r0, seed, rounds, chains)
     assert Lg == bl[g] and rg.tolist() == best[g].tolist() and rlg.tolist() == hist
+
+
+@pytest.mark.parametrize("isa", ["native", "avx2", "scalar"])
+def test_exact_simd_sweep_equals_oracle_step(isa, monkeypatch):
+    # the CPU baseline's exact SIMD sweep returns the oracle's move, in every
+    # instruction-set variant, for every metric, thread count and tie pattern
+    if isa != "native":
+        monkeypatch.setenv("ORC_FAST_ISA", isa)
+    for n, side in [(4, 10), (5, 3), (7, 2), (9, 2), (17, 5), (40, 30), (300, 10**4), (900, 50), (1500, 2**22 - 1)]:
+        for metric in (0, 1, 2):
+            xy = O.gen_uniform(n, side, n + metric)
+            for seed in range(3):
+                r = O.random_perm(n, seed) if seed < 2 else O.nn_tour(xy)
+                for th in (1, 3):
+                    assert O.step_fast(xy, r, th, metric) == O.step(xy, r, th, metric)
