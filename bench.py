@@ -259,6 +259,52 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
+PLA33810_OPT = 66048945  # TSPLIB known optimum (src/utils/cut.c:299-300)
+
+
+def tsplib_pla33810(ils_chains, ils_rounds):
+    """The TSPLIB instance of config 2's size, pla33810 (CEIL_2D, shipped in the
+    reference's TSPLIB/ and committed as tests/golden/tsplib/pla33810.tsp.xz):
+    local search from the nearest-neighbour and multiple-fragment tours, then a
+    short multi-start ILS with a shared best from the MF optimum."""
+    import lzma
+    import tempfile
+
+    with open(os.path.join(ROOT, "tests", "golden", "tsplib", "pla33810.tsp.xz"), "rb") as f:
+        data = lzma.decompress(f.read())
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "pla33810.tsp")
+        with open(p, "wb") as f:
+            f.write(data)
+        info, coords = L.tsp_read(p)
+    n, metric = info["n"], info["metric"]
+    xy = np.ascontiguousarray(np.frombuffer(coords, dtype=np.float32).astype(np.int32))
+    out = {"instance": "pla33810 (TSPLIB, CEIL_2D)", "n": n, "known_optimum": PLA33810_OPT}
+    with L.Context(coords, metric=metric) as ctx:
+        for name, build in (("nn", lambda: L.nn_tour(xy, 0)), ("mf", lambda: L.mf_tour(xy))):
+            t0 = time.perf_counter()
+            start = build()
+            tc = time.perf_counter() - t0
+            s0 = ctx.stats()
+            t0 = time.perf_counter()
+            lo = ctx.optimize(start, sync_every=64)
+            dt = time.perf_counter() - t0
+            s1 = ctx.stats()
+            out[name] = {"construction_s": tc, "start_length": L.host_route_length(coords, start, metric),
+                         "local_search_s": dt, "moves": lo["moves"], "length": lo["length"],
+                         "gap_pct": 100.0 * (lo["length"] / PLA33810_OPT - 1.0),
+                         "evals_per_s": (s1["evals"] - s0["evals"]) / dt}
+            if name == "mf":
+                mf_lo = lo["route"]
+        t0 = time.perf_counter()
+        r, best, rl = ctx.ils_multistart(mf_lo, 1, ils_rounds, ils_chains)
+        dt = time.perf_counter() - t0
+        out["ils_multistart"] = {"chains": ils_chains, "rounds": ils_rounds, "seconds": dt, "length": best,
+                                 "gap_pct": 100.0 * (best / PLA33810_OPT - 1.0),
+                                 "valid": L.host_route_length(coords, r, metric) == best}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,6 +321,9 @@ def main():
     ap.add_argument("--serve-horizon", type=float, default=12.0)
     ap.add_argument("--serve-max-running", type=int, default=0)
     ap.add_argument("--serve-max-pairs", type=float, default=0, help="cap on in-flight candidates per sweep")
+    ap.add_argument("--skip-tsplib", action="store_true")
+    ap.add_argument("--tsplib-chains", type=int, default=16)
+    ap.add_argument("--tsplib-rounds", type=int, default=100)
     ap.add_argument("--config3", action="store_true", help="also run the 100k-city ILS (long)")
     ap.add_argument("--config3-rounds", type=int, default=500)
     ap.add_argument("--config3-start", default="nn", choices=["nn", "random"])
@@ -353,6 +402,9 @@ def main():
         rates = [float(v) for v in args.serve_rates.split(",")]
         res["serving"] = serving(rates, args.serve_window, args.serve_horizon, max_running=args.serve_max_running,
                                  max_inflight_pairs=int(args.serve_max_pairs))
+    if not args.skip_tsplib:
+        log("TSPLIB pla33810")
+        res["tsplib"] = tsplib_pla33810(args.tsplib_chains, args.tsplib_rounds)
     if args.config3:
         log("config3 ILS 100k")
         res["ils_100k"] = config3(args.config3_rounds, args.config3_start)
