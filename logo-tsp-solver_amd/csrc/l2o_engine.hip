@@ -1196,7 +1196,6 @@ int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32
                  uint64_t *best_length, uint64_t *round_lengths)
 {
     if (!ctx || !route_inout || (kick != 0 && kick != 1)) return LOGO2OPT_EINVAL;
-    if (rounds == 0) rounds = 0;
     logo2opt_engine *e = ctx->e;
     if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
     logo2opt_job job{};

@@ -150,3 +150,25 @@ def test_pla_file_ceil2d(name, moves_checked):
         assert out["length"] == O.route_length(xy, out["route"], O.METRIC_CEIL_2D)
         assert not O.step(xy, out["route"], THREADS, metric=O.METRIC_CEIL_2D)[0]
         assert KNOWN_OPT[name] <= out["length"] <= 1.1 * KNOWN_OPT[name]
+
+
+def test_server_requests_carry_their_metric():
+    # requests of all three metrics in one continuous batch, each equal to the
+    # oracle's ILS under its own metric
+    reqs = []
+    for k in range(9):
+        n = [150, 420, 900][k % 3]
+        xy = O.gen_uniform(n, 10**4, 300 + k)
+        reqs.append(dict(n=n, xy=xy, coords=L.coords_of(xy), route=O.random_perm(n, 400 + k), seed=500 + k,
+                         metric=k % 3, rounds=5))
+    with L.Server(hbm_budget=128 << 20, max_slots=16) as srv:
+        ids = [srv.submit(q["coords"], q["route"], ils_rounds=q["rounds"], seed=q["seed"], metric=q["metric"])
+               for q in reqs]
+        for _ in range(100000):
+            running, queued = srv.pump(steps=64, sync_every=16)
+            if running == 0 and queued == 0:
+                break
+        for rid, q in zip(ids, reqs):
+            got = srv.result(rid, q["n"])
+            best, Lb, _, _ = O.ils(q["xy"], q["route"], q["seed"], q["rounds"], threads=THREADS, metric=q["metric"])
+            assert got["length"] == Lb and got["route"].tolist() == best.tolist()
