@@ -150,18 +150,48 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 // ---------------------------------------------------------------------------
 // sweep
 // ---------------------------------------------------------------------------
-// A work item covers rows a in [a0, a0+T) and columns b in [b0, b0+64*WAVES*K).
-// Lane l of wave w owns the K columns [bl, bl+K), bl = b0 + (64w + l) K, and
-// keeps the K+1 points P[bl .. bl+K] in registers; rows stream through the
-// scalar cache (s_load, uniform addresses), so the loop reads no LDS and no
-// vector memory.  Per step and lane: K+1 values V (2(K+1) v_sad_u32), K adds,
-// a min tree (v_min3) and one compare, ballotted into an SGPR mask.  Waves are
-// independent (no LDS, no barrier): each reduces its own key.
+// A work item covers rows a in [a0, a0+T) and columns b in
+// [b0, b0 + WAVES (64K - 1)).  Wave w starts at bw = b0 + w (64K - 1); lane l
+// keeps the K+1 points P[bl .. bl+K], bl = bw + lK, in registers and owns the
+// candidate columns [bl, bl+K) -- except lane 63's last one, which is the
+// next wave's first.  Rows stream through the scalar cache (s_load, uniform
+// addresses), so the loop reads no LDS and no vector memory.
+//
+// The offsets form one alternating chain over the wave's columns
+// (o_i + o_{i+1} = -W of column i+1), so V(a, k+1) of lane l's column K is
+// V(a, 0) of lane l+1: a candidate's second term for k = K-1 comes from the
+// next lane through DPP (v_add_u32_dpp wave_shl:1) instead of two more
+// v_sad_u32.  Per step and lane: K values V (2K v_sad_u32), K adds (one with
+// DPP), a min tree (v_min3) and one compare, ballotted into an SGPR mask.
+// Lane 63 has no next lane (DPP reads 0 there); the chain's free start is set
+// so that its o of column K-1 is 2^30, which keeps that candidate -- owned by
+// the next wave -- out of tier 1.  Waves are independent (no LDS, no
+// barrier): each reduces its own key.
 //
 // Invalid candidates never pass tier 1: a column b > n-2 gets +2^30 in its
 // offset pair, and in a wave that meets the diagonal (some b < a+2) each
 // candidate is masked by its own step limit.
 #define L2O_T1_BIG 0x40000000u
+
+// DPP wave_shl:1 -- lane l reads lane l+1; lane 63 reads 0 (bound_ctrl)
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);
+}
+
+// inclusive prefix sum over the 64 lanes (mod 2^32) with DPP: shifts by 1, 2,
+// 4, 8 within each row of 16 lanes (zero-filled at the row start), then the
+// last lane of row 0 / rows 0-1 broadcast into the rows above
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false); // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false); // row_bcast:31
+    return v;
+}
 
 // K = 8 needs 88 VGPRs unconstrained (5 waves per SIMD); asking for 6 waves
 // makes the allocator spill 68 bytes outside the hot loops instead
@@ -192,7 +222,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     const int a0 = (int)it.a0;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int bw = (int)it.b0 + w * 64 * K;  // first column of this wave
+    constexpr int WC = 64 * K - 1;             // candidate columns of a wave
+    const int bw = (int)it.b0 + w * WC;       // first column of this wave
     if (bw > n - 2) return;                   // wave-uniform: no candidate column
     const int bl = bw + lane * K;
     // rows a <= n-4 have candidates
@@ -200,20 +231,44 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     // a wave whose first column reaches past the last row + 2 never meets the
     // diagonal b = a+1; otherwise rows past its last column - 2 are empty
     const bool diag = bw < a0 + t_end + 1;
-    if (diag) t_end = min(t_end, min(bw + 64 * K - 1, n - 2) - 1 - a0);
+    if (diag) t_end = min(t_end, min(bw + WC - 1, n - 2) - 1 - a0);
 
-    // lane columns and offsets
+    // lane columns and offsets.  Wn[j] = W of column bl+j, minus 2^30 when
+    // the candidate column bl+j-1 lies beyond n-2 (its pair sum gets +2^30).
+    // (W first, coordinates after the offsets: fewer registers live at once)
     uint32_t cx[K + 1], cy[K + 1], o[K + 1];
+    {
+        uint32_t Wn[K + 1];
+#pragma unroll
+        for (int j = 0; j <= K; ++j)
+            Wn[j] = (uint32_t)P[min(bl + j, n - 1)].w - (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
+        // The chain o_i = (-1)^i (o_0 - A_i), A_i = sum_{t=1..i} (-1)^t W_t
+        // over the wave's columns i = lK + j: A at each lane's first column is
+        // an exclusive scan of the lanes' signed sums; o_0 makes o of column
+        // 64K-1 (lane 63, k = K-1) equal 2^30.
+        const uint32_t sgn_lane = ((lane * K) & 1) ? 0xFFFFFFFFu : 1u;  // (-1)^{lK}
+        uint32_t csum = 0;
+#pragma unroll
+        for (int j = 1; j <= K; ++j) csum += (j & 1) ? 0u - Wn[j] : Wn[j];
+        csum *= sgn_lane;
+        const uint32_t incl = wave_inclusive_scan(csum);
+        const uint32_t excl = incl - csum;                                                // A_{lK}
+        const uint32_t a_all = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);       // A_{64K}
+        const uint32_t w_last = (uint32_t)__builtin_amdgcn_readlane((int)Wn[K], 63);     // W_{64K}, sign +
+        const uint32_t o_first = a_all - w_last - L2O_T1_BIG;               // o_0 of the wave
+        o[0] = (o_first - excl) * sgn_lane;
+#pragma unroll
+        for (int j = 1; j <= K; ++j) o[j] = 0u - Wn[j] - o[j - 1];
+        // opaque from here: otherwise the compiler cancels o_k + o_{k+1}
+        // inside a candidate's sum and rewrites v_sad_u32 as max/min/sub
+#pragma unroll
+        for (int j = 0; j <= K; ++j) asm volatile("" : "+v"(o[j]));
+    }
 #pragma unroll
     for (int j = 0; j <= K; ++j) {
-        const l2o_pt c = ld_pt(P, (uint32_t)min(bl + j, n - 1));
-        cx[j] = (uint32_t)c.x;
-        cy[j] = (uint32_t)c.y;
-        // only o_k + o_{k+1} matters; o_0 = W of the first column keeps the
-        // first pair of v_sad_u32 an accumulate (a constant 0 would be
-        // lowered to max - min)
-        if (j == 0) o[0] = (uint32_t)c.w;
-        else o[j] = 0u - (uint32_t)c.w - o[j - 1] + (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
+        const uint32_t c = (uint32_t)min(bl + j, n - 1);
+        cx[j] = (uint32_t)P[c].x;
+        cy[j] = (uint32_t)P[c].y;
     }
     // diagonal waves: candidate (a0+t, bl+k) exists iff t - lim0 <= k
     const int lim0 = bl - 2 - a0;
@@ -268,7 +323,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
             for (int k = 0; k < K; ++k) {
                 const uint32_t va = sad((uint32_t)ya, cy[k], sad((uint32_t)xa, cx[k], o[k]));
                 const uint32_t vb = sad((uint32_t)yb, cy[k + 1], sad((uint32_t)xb, cx[k + 1], o[k + 1]));
-                if ((int)(va + vb) <= u && (!diag || t - lim0 <= k)) {
+                // lane 63's last column belongs to the next wave
+                if ((int)(va + vb) <= u && (!diag || t - lim0 <= k) && (k + 1 < K || lane != 63)) {
                     const int dc = P[bl + k + 1].d;
                     const float w2 = (est_dist(xa, ya, (int)cx[k], (int)cy[k], es, ea) +
                                       est_dist(xb, yb, (int)cx[k + 1], (int)cy[k + 1], es, ea)) - (float)dc;
@@ -312,9 +368,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
         for (int s = 0; s < NS; ++s) {
             const uint32_t rx = (uint32_t)r[s][0], ry = (uint32_t)r[s][1];
             const int u = tb + r[s][2];
-            uint32_t Vn[K + 1];
+            uint32_t Vn[K];
 #pragma unroll
-            for (int j = 0; j <= K; ++j) {
+            for (int j = 0; j < K; ++j) {
                 Vn[j] = sad(ry, cy[j], sad(rx, cx[j], o[j]));
                 // keeps each V a pair of v_sad_u32 (the last one would
                 // otherwise be reassociated into max/min/sub/add3)
@@ -323,7 +379,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
             int m = 0x7FFFFFFF;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                int S = (int)(V[k] + Vn[k + 1]);
+                // column K of this lane is column 0 of the next lane
+                int S = (int)(V[k] + (k + 1 < K ? Vn[k + 1] : from_next_lane(Vn[0])));
                 if constexpr (MASK) S = (t + s - lim0 <= k) ? S : 0x7FFFFFFF;
                 m = min(m, S);
             }

@@ -54,13 +54,14 @@ __host__ __device__ inline int32_t ceil_sqrtg(int32_t d, int32_t g)
 __host__ __device__ inline int32_t ceil_sqrt2(int32_t d) { return ceil_sqrtg(d, 2); }
 
 // Sweep tile shape: each workgroup has `waves` wave64s, each lane owns `k`
-// adjacent candidate columns for `t` rows.  A workgroup covers 64*waves*k
-// columns.
+// adjacent candidate columns for `t` rows (the last lane of a wave one fewer:
+// its last column is the next wave's first).  A workgroup covers
+// waves*(64k-1) columns.
 struct sweep_cfg {
     int k;
     int t;
     int waves;
-    int cols() const { return 64 * waves * k; }
+    int cols() const { return waves * (64 * k - 1); }
 };
 
 bool sweep_cfg_supported(const sweep_cfg &c);
