@@ -504,6 +504,8 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
     }
     l2o_pt p[PER], q[PER];
     uint32_t rid[PER];
+    // the far ends of the two new edges, loaded with the rest (one round trip)
+    const l2o_pt endJ = ld_pt(P, J), endJ1 = ld_pt(P, J - 1);
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
         p[e] = ld_pt(P, src[e]);
@@ -517,7 +519,7 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
         l2o_pt r = p[e];
         if (k == 0 || k == L) {
             // the two new edges: P[i] -> P[J] at position 0, P[J-1] -> P[i-1] at L
-            const l2o_pt o = ld_pt(P, k == 0 ? J : J - 1);
+            const l2o_pt o = k == 0 ? endJ : endJ1;
             r.d = exact_dist(r.x, r.y, o.x, o.y, metric);
             r.w = ceil_sqrtg(r.d, tier1_g(metric));
         } else {
@@ -527,8 +529,15 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
         st_pt(Pn, k, r);
         Rn[k] = rid[e];
     }
-    __threadfence();
-    __syncthreads();
+    // Every wave of the workgroup has read key, parity and its sources once
+    // all are past the barrier (each loaded value feeds a store issued before
+    // it), so the last chunk to count may commit the move.  No memory fence:
+    // nothing in this kernel reads the new buffers, and the next kernel sees
+    // them at the kernel boundary; the empty asm keeps the compiler from
+    // moving memory operations across the barrier.
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
     if (threadIdx.x == 0) {
         const uint32_t nch = (n + L2O_APPLY_CHUNK - 1) / L2O_APPLY_CHUNK;
         const uint32_t prev = atomicAdd(&in->apply_done, 1u);
@@ -546,7 +555,6 @@ __global__ __launch_bounds__(L2O_APPLY_THREADS) void k_apply(const l2o_apply_ite
             in->sweeps += 1;
             in->parity = par ^ 1;
             in->key = L2O_SENTINEL;
-            __threadfence();
         }
     }
 }
