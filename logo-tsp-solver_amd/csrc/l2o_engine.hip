@@ -395,8 +395,14 @@ void drop_graph(logo2opt_engine *e)
     e->gsteps = 0;
 }
 
-// 1/L2O_TAIL_SPLIT of a launch's work, at its end, goes in quarter-height items
-#define L2O_TAIL_SPLIT 5
+// 1/L2O_TAIL_SPLIT of a launch's work, at its end, goes in items of
+// 1/L2O_TAIL_DIV of the shape's rows (T/L2O_TAIL_DIV must stay a multiple of 4)
+#ifndef L2O_TAIL_SPLIT
+#define L2O_TAIL_SPLIT 8
+#endif
+#ifndef L2O_TAIL_DIV
+#define L2O_TAIL_DIV 4
+#endif
 
 // candidates (a, b), a+2 <= b <= n-2, of a sweep item (closed form)
 uint64_t item_work(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
@@ -466,13 +472,14 @@ int rebuild_lists(logo2opt_engine *e)
         std::vector<l2o_sweep_item> sorted;
         sorted.reserve(v.size() + v.size() / 2);
         // The last L2O_TAIL_SPLIT of the work (the end of the dispatch) is
-        // issued as items of t/4 rows, so the launch drains in short pieces.
+        // issued as items of t/L2O_TAIL_DIV rows, so the launch drains in
+        // short pieces.
         uint64_t total = 0, done = 0;
         for (uint64_t w : work) total += w;
         for (size_t q = 0; q < idx.size(); ++q) {
             const l2o_sweep_item it = v[idx[q]];
-            if (c.t >= 16 && done >= total - total / L2O_TAIL_SPLIT) {
-                const uint32_t r4 = (uint32_t)c.t / 4;
+            if (c.t >= 4 * L2O_TAIL_DIV && done >= total - total / L2O_TAIL_SPLIT) {
+                const uint32_t r4 = (uint32_t)c.t / L2O_TAIL_DIV;
                 for (uint32_t r = 0; r < it.rows; r += r4) sorted.push_back({it.inst, it.a0 + r, it.b0, r4});
             } else {
                 sorted.push_back(it);
