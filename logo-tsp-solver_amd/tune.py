@@ -6,9 +6,8 @@ Writes two tables ("max_n k t waves" per line) that the library loads on first
 use from its own directory:
   tuning_gfx950.txt        one instance in flight (logo2opt_create contexts,
                            engines with fewer than 8 live instances): each
-                           shape is timed with HIP events (logo2opt_time_sweep)
-                           on the nearest-neighbour start tour and on the
-                           local optimum a search reaches from it
+                           shape is timed on the steps of a local search from
+                           the nearest-neighbour start tour
   tuning_batch_gfx950.txt  batches (8+ live instances): each shape is timed on
                            an engine holding enough instances of that size to
                            fill the GPU (logo2opt_engine_time_sweep), and the
@@ -20,6 +19,7 @@ import argparse
 import math
 import os
 import sys
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
@@ -35,43 +35,41 @@ def pairs(n):
 
 
 def tune_single(reps=REPS, seed=1, verbose=True):
-    """Times every shape on the two ends of a local search from the greedy
-    nearest-neighbour start -- the start tour and the 2-opt local optimum the
-    search reaches (also what ILS kicks perturb) -- on uniform instances and,
-    from 16,000 cities, clustered ones."""
+    """Times every shape on local-search steps from the greedy nearest-neighbour
+    start, in a one-instance engine (what logo2opt_optimize runs): the whole
+    search up to 40,000 cities, the first 400 steps above; uniform instances
+    and, from 16,000 cities, clustered ones.  The search is bit-identical for
+    every shape, so all shapes run the same steps; the first 64 (graph capture
+    included) are not timed.  Timing single sweeps on fixed tours instead
+    ranks shapes differently, because the key published during a search
+    tightens the first-tier threshold as the sweep proceeds."""
     best = []
-    default = L.tuning_get(L.TABLE_SINGLE)
     for n in reps:
         side = 10**4 if n <= 5000 else 10**6
         kinds = ["u", "c"] if n >= 16000 else ["u"]
         cases = []
         for kind in kinds:
             xy = L.gen_uniform(n, side, seed + n) if kind == "u" else L.gen_clustered(n, side, seed + n)
-            ctx = L.Context(L.coords_of(xy))
-            r_nn = L.nn_tour(xy, 0)
-            tours = [r_nn]
-            if n <= 40000:  # the local optimum (a few seconds at most)
-                L.tuning_set(default, L.TABLE_SINGLE)
-                tours.append(ctx.optimize(r_nn, sync_every=64)["route"])
-            cases.append((ctx, tours))
-        sweeps = max(5, min(50, int(5e9 / (n * n))))
+            cases.append((L.coords_of(xy), L.nn_tour(xy, 0)))
+        steps = 0xFFFFFFFF if n <= 40000 else 400
         times = {}
         for sh in L.tuning_shapes():
             L.tuning_set([(L.MAX_N, sh)], L.TABLE_SINGLE)
             tot = 0.0
-            for ctx, tours in cases:
-                for t in tours:
-                    ctx.time_sweep(t, 1)  # first launch of this shape
-                    tot += ctx.time_sweep(t, sweeps)
+            for coords, r0 in cases:
+                with L.Engine(hbm_budget=L.Engine.bytes_for(n) + (16 << 20), max_slots=1) as eng:
+                    eng.add(coords, r0)
+                    if eng.run(max_steps=64, sync_every=64) == 0:
+                        continue  # converged during the warm-up: nothing to time
+                    t0 = time.perf_counter()
+                    eng.run(max_steps=steps, sync_every=64)
+                    tot += time.perf_counter() - t0
             times[sh] = tot
-        for ctx, _ in cases:
-            ctx.close()
         b = min(times, key=times.get)
         best.append((n, b))
         if verbose:
-            ncases = sum(len(tours) for _, tours in cases)
             ranked = sorted(times.items(), key=lambda kv: kv[1])[:4]
-            print(f"single n={n:>6}: " + "  ".join(f"{s}:{ms / ncases * 1e3:.1f}us" for s, ms in ranked), flush=True)
+            print(f"single n={n:>6}: " + "  ".join(f"{s}:{t * 1e3:.1f}ms" for s, t in ranked), flush=True)
     return best
 
 
