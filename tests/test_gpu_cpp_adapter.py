@@ -1,0 +1,84 @@
+"""The C++ drop-in path end to end on the GPU: a program compiled against the
+header-only adapter (logo-tsp-solver_amd/csrc/hip_local_solver.h, the
+reference's LocalSolver method names) reads a TSPLIB file with
+readFileCoords, runs optimizeStep / optimize / ils / routeLength through
+HIPSolver, writes the tour with saveRouteToAFile and reads it back with
+routeInitFromFile; its numbers must equal the oracle's."""
+import lzma
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import logo2opt as L
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+PROGRAM = r'''
+#include "hip_local_solver.h"
+#include <cstdio>
+int main(int argc, char **argv)
+{
+    int metric = -1;
+    std::vector<logo2opt_city_coords> c = logo2opt::readFileCoords(argv[1], &metric);
+    const uint32_t n = (uint32_t)c.size();
+    std::vector<uint32_t> route;
+    logo2opt::routeInitFromFile(route, n, argv[2]);
+    logo2opt::HIPSolver s(c.data(), n, 0, 64, metric);
+    const logo2opt_best2_out first = s.optimizeStep(route);
+    std::printf("step %u %u %d\n", first.i, first.j, first.minchange);
+    std::vector<unsigned long> best;
+    s.optimize(route, best);
+    std::printf("lo %llu %llu\n", (unsigned long long)s.routeLength(route), (unsigned long long)best.back());
+    const uint64_t ils = s.ils(route, 7, 10);
+    std::printf("ils %llu %llu\n", (unsigned long long)ils, (unsigned long long)s.routeLength(route));
+    logo2opt::saveRouteToAFile(route, argv[3], (unsigned long)ils);
+    std::vector<uint32_t> back;
+    logo2opt::routeInitFromFile(back, n, (std::string(argv[3]) + "_" + std::to_string(ils) + ".tour").c_str());
+    std::printf("saved %d\n", (int)(back == route));
+    return 0;
+}
+'''
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", ["att532", "berlin52"])
+def test_cpp_adapter_on_tsplib_file(tmp_path, name):
+    pkg = os.path.dirname(L.LIB_PATH)
+    src = tmp_path / "drop_in.cpp"
+    src.write_text(PROGRAM)
+    exe = tmp_path / "drop_in"
+    subprocess.run(["g++", "-std=c++11", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(pkg, "csrc"), str(src), "-o", str(exe), "-L", pkg, "-llogo2opt",
+                    f"-Wl,-rpath,{pkg}"], check=True)
+    tsp = tmp_path / (name + ".tsp")
+    with open(os.path.join(GOLD, "tsplib", name + ".tsp.xz"), "rb") as f:
+        tsp.write_bytes(lzma.decompress(f.read()))
+    info, coords = L.tsp_read(str(tsp))
+    n = info["n"]
+    xy = np.ascontiguousarray(np.frombuffer(coords, dtype=np.float32).astype(np.int32))
+    r0 = O.random_perm(n, 11)
+    start = tmp_path / "start.tour"
+    L.tour_write(str(start), r0, headerless=True)  # the reference's own tour layout
+    out = subprocess.run([str(exe), str(tsp), str(start), str(tmp_path / "best")], capture_output=True, text=True,
+                         timeout=120)
+    say(out.stdout, out.stderr)
+    assert out.returncode == 0
+    lines = {l.split()[0]: [int(v) for v in l.split()[1:]] for l in out.stdout.strip().split("\n")}
+    m = info["metric"]
+    ok, (i, j, g) = O.step(xy, r0, 4, metric=m)
+    assert lines["step"] == [i, j, g]
+    r, Lo, *_ = O.local_search(xy, r0, threads=4, metric=m)
+    assert lines["lo"] == [Lo, Lo]
+    rb, Lb, _, _ = O.ils(xy, r, 7, 10, threads=4, metric=m)
+    assert lines["ils"] == [Lb, Lb]
+    assert lines["saved"] == [1]
