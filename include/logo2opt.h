@@ -154,6 +154,10 @@ int logo2opt_get_stats(logo2opt_ctx *ctx, logo2opt_stats *st);
  * graphs are not used while it is on. */
 int logo2opt_set_timing(logo2opt_ctx *ctx, int on);
 
+/* Row pruning for this context's sweeps, including its multi-start ILS
+ * chains; see logo2opt_engine_set_pruning. */
+int logo2opt_set_pruning(logo2opt_ctx *ctx, int on);
+
 /* Times `sweeps` back-to-back launches of the sweep kernel on `route`
  * (no apply) with HIP events on the context's stream: avg_ms receives the
  * average kernel duration.  Used for the roofline figure. */
@@ -235,6 +239,13 @@ int logo2opt_engine_set_timing(logo2opt_engine *e, int on);
 /* on = 0: issue every step as direct launches instead of replaying captured
  * HIP graphs (also LOGO2OPT_GRAPHS=0 in the environment at engine creation). */
 int logo2opt_engine_set_graphs(logo2opt_engine *e, int on);
+/* on = 1: sweeps skip the blocks of 4 rows in which no candidate can pass the
+ * integer bound (a row's L1 distance to the bounding box of a wave's columns
+ * already exceeds it).  The selected moves, tours and lengths are identical;
+ * fewer candidates are evaluated, so the sweep's evals/s figure does not
+ * apply.  Off by default.  No counterpart in the reference (its
+ * kernels evaluate every pair, opt_kernel.cu:34-262). */
+int logo2opt_engine_set_pruning(logo2opt_engine *e, int on);
 
 /* ======================================================================== */
 /* Server: continuous batching of local-search / ILS requests                */
@@ -267,6 +278,8 @@ int logo2opt_server_release(logo2opt_server *s, int64_t id);
  * summed over running requests (the head of the queue is always admitted
  * when nothing runs). */
 int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_t max_inflight_pairs);
+/* Row pruning for every request's sweeps; see logo2opt_engine_set_pruning. */
+int logo2opt_server_set_pruning(logo2opt_server *s, int on);
 /* counts[6] = {submitted, rejected, admitted, done, queued now, running now} */
 int logo2opt_server_counts(logo2opt_server *s, uint64_t *counts);
 /* Open-loop replay of `count` requests arriving at arrival_s[q] seconds

@@ -250,6 +250,9 @@ struct logo2opt_engine {
     // batches of steps are captured into a HIP graph unless disabled
     // (logo2opt_engine_set_graphs, or LOGO2OPT_GRAPHS=0 in the environment)
     bool graphs = true;
+    // sweeps skip the blocks of rows that the bounding-box bound rules out
+    // (logo2opt_engine_set_pruning; same moves, fewer candidates evaluated)
+    bool prune = false;
     std::vector<hipEvent_t> tev;
     uint32_t tev_used = 0;
 };
@@ -260,6 +263,7 @@ struct logo2opt_ctx {
     uint32_t ge_chains = 0;
     uint32_t n = 0;
     uint32_t metric = LOGO2OPT_METRIC_EUC_2D;
+    bool prune = false;  // applied to ge when it is created
     std::vector<logo2opt_city_coords> coords;
     int32_t slot = -1;
     uint64_t sweeps = 0, moves = 0, evals = 0, fallbacks = 0;
@@ -463,6 +467,10 @@ int rebuild_lists(logo2opt_engine *e)
         // partial tiles at the diagonal and the right and bottom edges), so
         // the dispatch tail is made of the short ones; stable, so equal items
         // keep their instance order
+        // (With row pruning the tiles near the diagonal keep most of their
+        // rows and the far ones are mostly skipped; the same order, which
+        // spreads the near-diagonal tiles over the launch, measured faster
+        // than issuing them first.)
         auto &v = kv.second;
         std::vector<uint64_t> work(v.size());
         for (size_t q = 0; q < v.size(); ++q) work[q] = item_work(v[q], c, e->slots[v[q].inst].n);
@@ -530,7 +538,7 @@ int issue_step(logo2opt_engine *e)
         L2O_TRY(hipEventRecord(e->tev[e->tev_used], e->stream));
     }
     for (const auto &g : e->groups) {
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
         e->sweep_launches += 1;
     }
     if (tm) {
@@ -1062,6 +1070,23 @@ int logo2opt_engine_set_graphs(logo2opt_engine *e, int on)
     return LOGO2OPT_OK;
 }
 
+int logo2opt_engine_set_pruning(logo2opt_engine *e, int on)
+{
+    if (!e) return LOGO2OPT_EINVAL;
+    e->prune = on != 0;
+    drop_graph(e);
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_set_pruning(logo2opt_ctx *ctx, int on)
+{
+    if (!ctx) return LOGO2OPT_EINVAL;
+    int rc = logo2opt_engine_set_pruning(ctx->e, on);
+    if (!rc && ctx->ge) rc = logo2opt_engine_set_pruning(ctx->ge, on);
+    if (!rc) ctx->prune = on != 0;
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
 // single-instance context
 // ---------------------------------------------------------------------------
@@ -1148,7 +1173,7 @@ int logo2opt_step(logo2opt_ctx *ctx, const uint32_t *route, logo2opt_best2_out *
     const double h1 = now_ms();
     L2O_TRY(hipEventRecord(e->ev0, e->stream));
     for (const auto &g : e->groups)
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
     L2O_TRY(hipEventRecord(e->ev1, e->stream));
     L2O_TRY(hipEventSynchronize(e->ev1));
     float ms = 0.0f;
@@ -1255,6 +1280,7 @@ int logo2opt_ils_multistart(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t s
         rc = engine_create(ctx->e->device, chains * bytes_for(ctx->n, &job) + (1ull << 24), chains, &ctx->ge);
         if (rc) return rc;
         ctx->ge_chains = chains;
+        ctx->ge->prune = ctx->prune;
     }
     logo2opt_engine *e = ctx->ge;
     for (uint32_t k = 0; k < e->max_slots; ++k)
@@ -1340,11 +1366,11 @@ int logo2opt_engine_time_sweep(logo2opt_engine *e, uint32_t sweeps, double *avg_
     int rc = rebuild_lists(e);
     if (rc) return rc;
     for (const auto &g : e->groups)  // untimed first launch
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
     L2O_TRY(hipEventRecord(e->ev0, e->stream));
     for (uint32_t s = 0; s < sweeps; ++s)
         for (const auto &g : e->groups)
-            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
     L2O_TRY(hipEventRecord(e->ev1, e->stream));
     L2O_TRY(hipEventSynchronize(e->ev1));
     float ms = 0.0f;
@@ -1381,11 +1407,11 @@ int logo2opt_time_sweep(logo2opt_ctx *ctx, const uint32_t *route, uint32_t sweep
     if (rc) return rc;
     // one untimed launch (code object load, caches)
     for (const auto &g : e->groups)
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
     L2O_TRY(hipEventRecord(e->ev0, e->stream));
     for (uint32_t s = 0; s < sweeps; ++s)
         for (const auto &g : e->groups)
-            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream));
+            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
     L2O_TRY(hipEventRecord(e->ev1, e->stream));
     L2O_TRY(hipEventSynchronize(e->ev1));
     float ms = 0.0f;

@@ -193,14 +193,40 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v)
     return v;
 }
 
+// maximum over the 64 lanes, returned wave-uniform: the scan pattern above
+// with max, where a lane that a shift leaves without a source keeps its value
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Row pruning (PRUNE, opt-in).  Every candidate (a, b) of a wave has P[b] and
+// P[b+1] inside the wave's bounding box B of its column points and
+// W[b+1] <= Wmax, the wave's largest column W, so its tier-1 sum is at least
+//   lb(P[a]) + lb(P[a+1]) - Wmax,   lb(p) = L1 distance from p to B.
+// A row whose bound exceeds tier 1's threshold tb + W[a+1] has no tier-1 hit;
+// tb only falls during a sweep, so rows ruled out with the threshold at the
+// wave's start stay ruled out.  Before its row loop each wave tests its rows
+// with one lane per block of 4 rows and skips the blocks with no row left.
+// A skipped block would have had no tier-1 hit, so the candidates that reach
+// the exact pass, and the selected move, are those of the full sweep.  After
+// a skipped run of blocks V is recomputed from the next block's first row.
+//
 // K = 8 needs 88 VGPRs unconstrained (5 waves per SIMD); asking for 6 waves
 // makes the allocator spill 68 bytes outside the hot loops instead
-template <int K, int T, int WAVES>
+template <int K, int T, int WAVES, bool PRUNE>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >= 8 ? 6 : 1))) void k_sweep(const l2o_sweep_item *__restrict__ items,
                                                       l2o_inst *__restrict__ insts)
 {
     constexpr int UF = 4;  // steps per unrolled iteration
     static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
+    static_assert(!PRUNE || T <= 64 * UF, "pruning tests one block of rows per lane");
     const l2o_sweep_item it = items[blockIdx.x];
     l2o_inst *in = insts + it.inst;
     // The fields this kernel reads but no wave writes come in one batch of
@@ -237,11 +263,17 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     // the candidate column bl+j-1 lies beyond n-2 (its pair sum gets +2^30).
     // (W first, coordinates after the offsets: fewer registers live at once)
     uint32_t cx[K + 1], cy[K + 1], o[K + 1];
+    int wmax = 0;  // PRUNE: the lane's, then the wave's largest W of a column b+1
     {
         uint32_t Wn[K + 1];
 #pragma unroll
         for (int j = 0; j <= K; ++j)
             Wn[j] = (uint32_t)P[min(bl + j, n - 1)].w - (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
+        // (beyond n-2 Wn is negative as an int and drops out)
+        if constexpr (PRUNE) {
+#pragma unroll
+            for (int j = 1; j <= K; ++j) wmax = max(wmax, (int)Wn[j]);
+        }
         // The chain o_i = (-1)^i (o_0 - A_i), A_i = sum_{t=1..i} (-1)^t W_t
         // over the wave's columns i = lK + j: A at each lane's first column is
         // an exclusive scan of the lanes' signed sums; o_0 makes o of column
@@ -269,6 +301,23 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
         const uint32_t c = (uint32_t)min(bl + j, n - 1);
         cx[j] = (uint32_t)P[c].x;
         cy[j] = (uint32_t)P[c].y;
+    }
+    // PRUNE: the wave's bounding box [bx0, bx1] x [by0, by1] of its column points
+    int bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
+    if constexpr (PRUNE) {
+        int mx = (int)cx[0], Mx = mx, my = (int)cy[0], My = my;
+#pragma unroll
+        for (int j = 1; j <= K; ++j) {
+            mx = min(mx, (int)cx[j]);
+            Mx = max(Mx, (int)cx[j]);
+            my = min(my, (int)cy[j]);
+            My = max(My, (int)cy[j]);
+        }
+        bx0 = -wave_max_i32(-mx);
+        bx1 = wave_max_i32(Mx);
+        by0 = -wave_max_i32(-my);
+        by1 = wave_max_i32(My);
+        wmax = wave_max_i32(wmax);
     }
     // diagonal waves: candidate (a0+t, bl+k) exists iff t - lim0 <= k
     const int lim0 = bl - 2 - a0;
@@ -303,6 +352,30 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     {
         const unsigned long long g = key_now();
         if (g != L2O_SENTINEL && key_gain(g) < cap) set_cap(key_gain(g));
+    }
+    // PRUNE: bit q set when block q (rows a0+4q .. a0+4q+3, the last block
+    // cut at t_end) holds a row that the bound does not rule out
+    unsigned long long live = 0;
+    if constexpr (PRUNE) {
+        auto lb = [&](const l2o_pt &p) __attribute__((always_inline)) {
+            return max(max(bx0 - p.x, p.x - bx1), 0) + max(max(by0 - p.y, p.y - by1), 0);
+        };
+        const int r0 = lane * UF;
+        bool keep = false;
+        if (r0 < t_end) {
+            // rows up to a0 + t_end <= n - 3
+            int lp = lb(ld_pt(P, (uint32_t)(a0 + r0)));
+#pragma unroll
+            for (int s = 0; s < UF; ++s) {
+                if (r0 + s < t_end) {
+                    const l2o_pt q = ld_pt(P, (uint32_t)(a0 + r0 + s + 1));
+                    const int lq = lb(q);
+                    keep = keep || lp + lq - wmax <= tb + q.w;
+                    lp = lq;
+                }
+            }
+        }
+        live = __ballot(keep);
     }
     unsigned long long lbest = L2O_SENTINEL;
     unsigned long long published = L2O_SENTINEL;
@@ -427,8 +500,63 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
             if (block(t, r1, std::integral_constant<int, 1>{}, mask_tag)) slow(t, 1);
         }
     };
-    if (diag) run(std::true_type{});
-    else run(std::false_type{});
+    // PRUNE: live blocks only.  Each buffer holds a block's first row and its
+    // four step rows; the next live block's rows are requested before this
+    // one computes.
+    auto run_pruned = [&](auto mask_tag) __attribute__((always_inline)) {
+        using UFc = std::integral_constant<int, UF>;
+        using UF1c = std::integral_constant<int, UF + 1>;
+        const int nmain = t_end / UF;  // whole blocks
+        unsigned long long todo = live & (nmain >= 64 ? ~0ull : (1ull << nmain) - 1);
+        int tv = 0;  // V holds row a0+tv
+        auto reseat = [&](int t, int x, int y) __attribute__((always_inline)) {
+            if (t != tv) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) V[k] = sad((uint32_t)y, cy[k], sad((uint32_t)x, cx[k], o[k]));
+            }
+        };
+        auto process = [&](int q, const int (*r)[3]) __attribute__((always_inline)) {
+            const int t = q * UF;
+            reseat(t, r[0][0], r[0][1]);
+            if (block(t, r + 1, UFc{}, mask_tag)) slow(t, UF);
+            tv = t + UF;
+        };
+        auto next = [&]() __attribute__((always_inline)) -> int {
+            if (!todo) return -1;
+            const int q = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            return q;
+        };
+        int ra[UF + 1][3], rb[UF + 1][3];
+        int qa = next();
+        if (qa >= 0) load_rows(qa * UF - 1, ra, UF1c{});  // rows 4qa .. 4qa+4
+        while (qa >= 0) {
+            const int qb = next();
+            if (qb >= 0) load_rows(qb * UF - 1, rb, UF1c{});
+            process(qa, ra);
+            if (qb < 0) break;
+            qa = next();
+            if (qa >= 0) load_rows(qa * UF - 1, ra, UF1c{});
+            process(qb, rb);
+        }
+        // the rows after the last whole block, if the bound left any
+        int t = nmain * UF;
+        if (t < t_end && ((live >> nmain) & 1)) {
+            reseat(t, prow[4 * t], prow[4 * t + 1]);
+            for (; t < t_end; ++t) {
+                int r1[1][3];
+                load_rows(t, r1, std::integral_constant<int, 1>{});
+                if (block(t, r1, std::integral_constant<int, 1>{}, mask_tag)) slow(t, 1);
+            }
+        }
+    };
+    if constexpr (PRUNE) {
+        if (diag) run_pruned(std::true_type{});
+        else run_pruned(std::false_type{});
+    } else {
+        if (diag) run(std::true_type{});
+        else run(std::false_type{});
+    }
 
     // every improvement was published by the slow path
     const unsigned wfb = wave_sum_u32(fb);
@@ -830,12 +958,15 @@ namespace l2o {
 
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \
-        hipLaunchKernelGGL((k_sweep<K_, T_, W_>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
+        if (prune)                                                                                   \
+            hipLaunchKernelGGL((k_sweep<K_, T_, W_, true>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_sweep<K_, T_, W_, false>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
         return hipGetLastError();                                                                    \
     }
 
 hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint32_t nitems,
-                        l2o_inst *insts, hipStream_t stream)
+                        l2o_inst *insts, hipStream_t stream, bool prune)
 {
     if (nitems == 0) return hipSuccess;
     L2O_SWEEP_SHAPES(L2O_SWEEP_CASE)

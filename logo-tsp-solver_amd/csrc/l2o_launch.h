@@ -67,9 +67,10 @@ struct sweep_cfg {
 bool sweep_cfg_supported(const sweep_cfg &c);
 // compiled shapes: fills up to cap rows {k, t, waves}, returns the total count
 int sweep_shapes(int (*out)[3], int cap);
-// one workgroup per item
+// one workgroup per item; prune selects the variant that skips blocks of
+// rows a bounding-box bound rules out (same results)
 hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint32_t nitems,
-                        l2o_inst *insts, hipStream_t stream);
+                        l2o_inst *insts, hipStream_t stream, bool prune = false);
 hipError_t launch_apply(const l2o_apply_item *items, uint32_t nitems, l2o_inst *insts, unsigned *live,
                         hipStream_t stream);
 hipError_t launch_ils(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStream_t stream);

@@ -229,6 +229,12 @@ int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_
     return LOGO2OPT_OK;
 }
 
+int logo2opt_server_set_pruning(logo2opt_server *s, int on)
+{
+    if (!s) return LOGO2OPT_EINVAL;
+    return logo2opt_engine_set_pruning(s->e, on);
+}
+
 int logo2opt_server_counts(logo2opt_server *s, uint64_t *counts)
 {
     if (!s || !counts) return LOGO2OPT_EINVAL;

@@ -34,7 +34,8 @@ EXPORTS = [
     "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
     "logo2opt_engine_time_sweep", "logo2opt_create_metric", "logo2opt_tsp_read", "logo2opt_tour_read",
     "logo2opt_tour_write", "logo2opt_host_route_length", "logo2opt_mf_tour", "logo2opt_ils_multistart",
-    "logo2opt_engine_add_group", "logo2opt_engine_group_result",
+    "logo2opt_engine_add_group", "logo2opt_engine_group_result", "logo2opt_set_pruning",
+    "logo2opt_engine_set_pruning",
 ]
 
 
@@ -116,6 +117,8 @@ def load():
     L.logo2opt_set_timing.argtypes = [vp, C.c_int]
     L.logo2opt_engine_set_timing.argtypes = [vp, C.c_int]
     L.logo2opt_engine_set_graphs.argtypes = [vp, C.c_int]
+    L.logo2opt_engine_set_pruning.argtypes = [vp, C.c_int]
+    L.logo2opt_set_pruning.argtypes = [vp, C.c_int]
     L.logo2opt_time_sweep.argtypes = [vp, u32p, C.c_uint32, C.POINTER(C.c_double)]
     L.logo2opt_engine_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(vp)]
     L.logo2opt_engine_destroy.argtypes = [vp]
@@ -369,6 +372,10 @@ class Context:
     def set_timing(self, on):
         _check(load().logo2opt_set_timing(self.h, 1 if on else 0), "logo2opt_set_timing")
 
+    def set_pruning(self, on):
+        """Skip blocks of rows the bounding-box bound rules out (same results)."""
+        _check(load().logo2opt_set_pruning(self.h, 1 if on else 0), "logo2opt_set_pruning")
+
     def time_sweep(self, route, sweeps):
         ms = C.c_double()
         _check(load().logo2opt_time_sweep(self.h, np.ascontiguousarray(route, np.uint32), sweeps, C.byref(ms)),
@@ -484,6 +491,9 @@ class Engine:
     def set_graphs(self, on):
         _check(load().logo2opt_engine_set_graphs(self.h, 1 if on else 0), "logo2opt_engine_set_graphs")
 
+    def set_pruning(self, on):
+        _check(load().logo2opt_engine_set_pruning(self.h, 1 if on else 0), "logo2opt_engine_set_pruning")
+
     def stats(self):
         s = Stats()
         _check(load().logo2opt_engine_stats(self.h, C.byref(s)), "logo2opt_engine_stats")
@@ -494,7 +504,7 @@ class Engine:
 SERVER_EXPORTS = ["logo2opt_server_create", "logo2opt_server_destroy", "logo2opt_server_submit",
                   "logo2opt_server_pump", "logo2opt_server_poll", "logo2opt_server_result",
                   "logo2opt_server_release", "logo2opt_server_counts", "logo2opt_server_replay",
-                  "logo2opt_server_set_limits"]
+                  "logo2opt_server_set_limits", "logo2opt_server_set_pruning"]
 EXPORTS += SERVER_EXPORTS
 
 
@@ -511,6 +521,7 @@ def _bind_server(L):
     L.logo2opt_server_release.argtypes = [vp, C.c_int64]
     L.logo2opt_server_counts.argtypes = [vp, u64p]
     L.logo2opt_server_set_limits.argtypes = [vp, C.c_uint32, C.c_uint64]
+    L.logo2opt_server_set_pruning.argtypes = [vp, C.c_int]
     L.logo2opt_server_replay.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_double, C.c_uint32, C.c_uint32,
                                          vp, vp, C.POINTER(C.c_double)]
     L._server_bound = True
@@ -571,6 +582,9 @@ class Server:
 
     def set_limits(self, max_running=0, max_inflight_pairs=0):
         _check(load().logo2opt_server_set_limits(self.h, max_running, max_inflight_pairs), "server_set_limits")
+
+    def set_pruning(self, on):
+        _check(load().logo2opt_server_set_pruning(self.h, 1 if on else 0), "server_set_pruning")
 
     def counts(self):
         c = np.zeros(6, np.uint64)
