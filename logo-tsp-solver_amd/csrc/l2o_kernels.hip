@@ -96,16 +96,32 @@ __device__ __forceinline__ int key_gain(unsigned long long key)
     return (int)((long long)(key >> 40) - L2O_KEY_GAIN_BIAS);
 }
 
+// one step of a 64-bit minimum scan over the lanes: DPP CTRL moves both
+// halves; lanes the move leaves without a source keep their own value
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned long long dpp_min_u64(unsigned long long v)
+{
+    const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    const uint32_t l2 = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xF, false);
+    const uint32_t h2 = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xF, false);
+    const unsigned long long o = ((unsigned long long)h2 << 32) | l2;
+    return o < v ? o : v;
+}
+
+// minimum over the 64 lanes, returned wave-uniform (in SGPRs): row_shr 1, 2,
+// 4, 8 within each row of 16 lanes, then row_bcast 15 and 31 carry the rows
+// below upwards, so lane 63 ends with the minimum (VALU only, no LDS)
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned lo = __shfl_xor((unsigned)v, off);
-        const unsigned hi = __shfl_xor((unsigned)(v >> 32), off);
-        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
-        v = o < v ? o : v;
-    }
-    return v;
+    v = dpp_min_u64<0x111, 0xF>(v);
+    v = dpp_min_u64<0x112, 0xF>(v);
+    v = dpp_min_u64<0x114, 0xF>(v);
+    v = dpp_min_u64<0x118, 0xF>(v);
+    v = dpp_min_u64<0x142, 0xA>(v);
+    v = dpp_min_u64<0x143, 0xC>(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 // a wave-uniform 64-bit value moved to SGPRs (readfirstlane returns int: each
@@ -349,7 +365,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
         tb = __builtin_amdgcn_readfirstlane(tier1_base(c + 1, g));
         thr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)c + slack)));
     };
-    {
+    // (PRUNE: read after the row test below, by waves with a live block only;
+    // the row test then uses cap = -1, a valid, looser threshold)
+    if constexpr (!PRUNE) {
         const unsigned long long g = key_now();
         if (g != L2O_SENTINEL && key_gain(g) < cap) set_cap(key_gain(g));
     }
@@ -376,6 +394,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
             }
         }
         live = __ballot(keep);
+        if (!live) return;  // nothing to evaluate, nothing published
+        const unsigned long long g = key_now();
+        if (g != L2O_SENTINEL && key_gain(g) < cap) set_cap(key_gain(g));
     }
     unsigned long long lbest = L2O_SENTINEL;
     unsigned long long published = L2O_SENTINEL;
@@ -416,11 +437,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
                 }
             }
         }
-        // the wave minimum is uniform: keep it, and the thresholds, in SGPRs;
-        // publish it at once so that other waves can tighten their cap
-        const unsigned long long wb = uniform_u64(wave_min_u64(lbest));
+        // a lane that found a key below the wave's last published one makes
+        // the wave reduce and publish its minimum at once (so that other
+        // waves can tighten their cap); the minimum and the thresholds are
+        // uniform and stay in SGPRs
         int c = cap;
-        if (wb < published) {
+        if (__ballot(lbest < published)) {
+            const unsigned long long wb = wave_min_u64(lbest);
             published = wb;
             c = min(c, key_gain(wb) - 1);
             if (lane == 0) atomicMin(&in->key, wb);
