@@ -42,9 +42,9 @@ METRIC = "2-opt move evals/sec + instances/sec batched vs plain, bit-exact tour 
 # does less (integer L1 bounds, see l2o_kernels.hip) and is priced the same
 FLOP_PER_EVAL = 14
 # HBM bytes per sweep launch of the 33,810-city search, from two rocprofv3 --pmc runs of this
-# benchmark (FETCH_SIZE 3,347.7 KiB x 2.0 + WRITE_SIZE 5.41 KiB x 1.0, factors from a 256 MiB copy;
+# benchmark (FETCH_SIZE 3,339.7 KiB x 2.0 + WRITE_SIZE 5.23 KiB x 1.0, factors from a 256 MiB copy;
 # profiles/k_sweep_config2_counters.md).  Reported only for the kernel that was profiled.
-PROFILED_TRAFFIC = {"l2o::k_sweep<6,160,1>": 3347.72 * 1024 * 2.0 + 5.41 * 1024 * 1.0}
+PROFILED_TRAFFIC = {"l2o::k_sweep<6,160,1,false>": 3339.72 * 1024 * 2.0 + 5.23 * 1024 * 1.0}
 REF_FLOP_PER_EVAL = 36    # the reference's GPU GFLOPS convention (solver.c:305-306)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 
@@ -61,7 +61,7 @@ def sweep_kernel_name(n, table=None):
     """Name of the sweep kernel instance the library launches for size n."""
     rows = L.tuning_get(L.TABLE_SINGLE if table is None else table)
     k, t, w = next((sh for mx, sh in rows if n <= mx), rows[-1][1])
-    return f"l2o::k_sweep<{k},{t},{w}>"
+    return f"l2o::k_sweep<{k},{t},{w},false>"
 
 
 def config2(seed=2):
