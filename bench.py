@@ -227,7 +227,7 @@ def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=51
             "rates": out}
 
 
-def config3(rounds, start, seed=3):
+def config3(rounds, start, seed=3, prune=True):
     """ILS on 100,000 uniform cities: LO, then `rounds` double-bridge kicks."""
     n = 100000
     xy = L.gen_uniform(n, 10**6, seed)
@@ -237,10 +237,19 @@ def config3(rounds, start, seed=3):
         r, best, rl = ctx.ils(r0, seed=seed + 2, rounds=rounds)
         dt = time.perf_counter() - t0
         st = ctx.stats()
-    return {"workload": f"ILS on 100,000 uniform cities in [0,1e6)^2, {start} start, {rounds} seeded "
-                        "double-bridge rounds, strict accept", "seconds": dt, "best_length": best,
-            "length_after_first_lo": int(rl[0]), "moves": st["moves"], "sweeps": st["sweeps"],
-            "evals_per_s": st["evals"] / dt, "fallbacks": st["fallbacks"]}
+        out = {"workload": f"ILS on 100,000 uniform cities in [0,1e6)^2, {start} start, {rounds} seeded "
+                           "double-bridge rounds, strict accept", "seconds": dt, "best_length": best,
+               "length_after_first_lo": int(rl[0]), "moves": st["moves"], "sweeps": st["sweeps"],
+               "evals_per_s": st["evals"] / dt, "fallbacks": st["fallbacks"]}
+        if prune:
+            # the same ILS with row pruning on: same tours, separate figure
+            ctx.set_pruning(True)
+            t0 = time.perf_counter()
+            rp, bp, rlp = ctx.ils(r0, seed=seed + 2, rounds=rounds)
+            dtp = time.perf_counter() - t0
+            assert bp == best and rlp.tolist() == rl.tolist() and rp.tolist() == r.tolist()
+            out["pruned"] = {"seconds": dtp, "speedup_vs_full": dt / dtp, "same_result": True}
+    return out
 
 
 def copy_calibration(nbytes):
@@ -430,7 +439,7 @@ def main():
         res["tsplib"] = tsplib_pla33810(args.tsplib_chains, args.tsplib_rounds)
     if args.config3:
         log("config3 ILS 100k")
-        res["ils_100k"] = config3(args.config3_rounds, args.config3_start)
+        res["ils_100k"] = config3(args.config3_rounds, args.config3_start, prune=not args.skip_pruned)
     print(json.dumps(res), flush=True)
 
 
