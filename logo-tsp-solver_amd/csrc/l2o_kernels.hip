@@ -275,6 +275,20 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     const bool diag = bw < a0 + t_end + 1;
     if (diag) t_end = min(t_end, min(bw + WC - 1, n - 2) - 1 - a0);
 
+    // PRUNE: the points of the row test below (lane l: rows a0+4l .. a0+4l+4),
+    // requested before the columns (their addresses do not depend on them);
+    // indices are clamped to a0 + t_end <= n - 3
+    int qx[PRUNE ? UF + 1 : 1], qy[PRUNE ? UF + 1 : 1], qw[PRUNE ? UF + 1 : 1];
+    if constexpr (PRUNE) {
+#pragma unroll
+        for (int s = 0; s <= UF; ++s) {
+            const uint32_t q = (uint32_t)min(a0 + lane * UF + s, a0 + max(t_end, 0));
+            qx[s] = P[q].x;
+            qy[s] = P[q].y;
+            qw[s] = P[q].w;
+        }
+    }
+
     // lane columns and offsets.  Wn[j] = W of column bl+j, minus 2^30 when
     // the candidate column bl+j-1 lies beyond n-2 (its pair sum gets +2^30).
     // (W first, coordinates after the offsets: fewer registers live at once)
@@ -375,23 +389,17 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >=
     // cut at t_end) holds a row that the bound does not rule out
     unsigned long long live = 0;
     if constexpr (PRUNE) {
-        auto lb = [&](const l2o_pt &p) __attribute__((always_inline)) {
-            return max(max(bx0 - p.x, p.x - bx1), 0) + max(max(by0 - p.y, p.y - by1), 0);
+        auto lb = [&](int x, int y) __attribute__((always_inline)) {
+            return max(max(bx0 - x, x - bx1), 0) + max(max(by0 - y, y - by1), 0);
         };
         const int r0 = lane * UF;
         bool keep = false;
-        if (r0 < t_end) {
-            // rows up to a0 + t_end <= n - 3
-            int lp = lb(ld_pt(P, (uint32_t)(a0 + r0)));
+        int lp = lb(qx[0], qy[0]);
 #pragma unroll
-            for (int s = 0; s < UF; ++s) {
-                if (r0 + s < t_end) {
-                    const l2o_pt q = ld_pt(P, (uint32_t)(a0 + r0 + s + 1));
-                    const int lq = lb(q);
-                    keep = keep || lp + lq - wmax <= tb + q.w;
-                    lp = lq;
-                }
-            }
+        for (int s = 0; s < UF; ++s) {
+            const int lq = lb(qx[s + 1], qy[s + 1]);
+            keep = keep || (r0 + s < t_end && lp + lq - wmax <= tb + qw[s + 1]);
+            lp = lq;
         }
         live = __ballot(keep);
         if (!live) return;  // nothing to evaluate, nothing published
