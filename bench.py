@@ -188,8 +188,9 @@ def batched_vs_plain(count, plain_every, seed=4):
 
 
 def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=512, max_running=0,
-            max_inflight_pairs=0):
-    """Open-loop Poisson replay through the continuous-batching server."""
+            max_inflight_pairs=0, prune=False):
+    """Open-loop Poisson replay through the continuous-batching server (the
+    same seeded pool and arrivals for every call with the same arguments)."""
     rs = np.random.default_rng(seed)
     insts = []
     for k in range(pool):
@@ -205,6 +206,7 @@ def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=51
         reqs = [(insts[p][1], insts[p][2], L.Job(rounds, 0, 7000 + q, 0, 0)) for q, p in enumerate(pick)]
         with L.Server(hbm_budget=8 << 30, max_slots=max_slots, queue_cap=1 << 20) as srv:
             srv.set_limits(max_running, max_inflight_pairs)
+            srv.set_pruning(prune)
             lat, qs, el = srv.replay(arrivals, reqs, horizon_s, steps=32, sync_every=16)
             cnt = srv.counts()
         ok = lat >= 0
@@ -330,6 +332,8 @@ def main():
     ap.add_argument("--serve-horizon", type=float, default=12.0)
     ap.add_argument("--serve-max-running", type=int, default=0)
     ap.add_argument("--serve-max-pairs", type=float, default=0, help="cap on in-flight candidates per sweep")
+    ap.add_argument("--no-serve-prune", action="store_true",
+                    help="skip the replay of the same arrivals with row pruning on (serving_pruned)")
     ap.add_argument("--skip-tsplib", action="store_true")
     ap.add_argument("--skip-pruned", action="store_true")
     ap.add_argument("--tsplib-chains", type=int, default=16)
@@ -434,6 +438,11 @@ def main():
         rates = [float(v) for v in args.serve_rates.split(",")]
         res["serving"] = serving(rates, args.serve_window, args.serve_horizon, max_running=args.serve_max_running,
                                  max_inflight_pairs=int(args.serve_max_pairs))
+        if not args.no_serve_prune:
+            log("serving, row pruning on")
+            res["serving_pruned"] = serving(rates, args.serve_window, args.serve_horizon,
+                                            max_running=args.serve_max_running,
+                                            max_inflight_pairs=int(args.serve_max_pairs), prune=True)
     if not args.skip_tsplib:
         log("TSPLIB pla33810")
         res["tsplib"] = tsplib_pla33810(args.tsplib_chains, args.tsplib_rounds)
