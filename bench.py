@@ -351,8 +351,9 @@ def main():
     if args.no_graph:
         os.environ["LOGO2OPT_GRAPHS"] = "0"
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); the sweep's "
+              "multi-GPU sharding (pair triangle split, one 8-byte min-allreduce per move) is not run.",
+              file=sys.stderr)
         sys.exit(2)
 
     n, xy, r0 = config2()
