@@ -375,8 +375,10 @@ int logo2opt_tsp_read(const char *path, uint32_t flags, logo2opt_tsp_info *info,
  * into route (0-based ids).  Accepts the TSPLIB form (header, TOUR_SECTION,
  * 1-based ids, terminated by -1 or EOF; a DIMENSION must equal n) and the
  * reference's own header-less form (n 1-based ids, one per line, as
- * saveRouteToAFile writes them).  Errors: LOGO2OPT_EIO, LOGO2OPT_EFORMAT
- * (wrong count, non-numeric entries, TYPE other than TOUR),
+ * saveRouteToAFile writes them; like routeInitFromFile, only the first n ids
+ * are read and anything after them is ignored).  Errors: LOGO2OPT_EIO,
+ * LOGO2OPT_EFORMAT (too few ids, or a TOUR_SECTION with more than n before
+ * its -1; non-numeric entries; TYPE other than TOUR),
  * LOGO2OPT_ETOUR (not a permutation). */
 int logo2opt_tour_read(const char *path, uint32_t n, uint32_t *route);
 

@@ -262,6 +262,13 @@ extern "C" int logo2opt_tour_read(const char *path, uint32_t n, uint32_t *route)
                 ended = true;  // the first tour only
                 break;
             }
+            // the header-less form reads the first n ids and ignores what
+            // follows, as routeInitFromFile's n fscanf calls do (a trailing
+            // "-1", say); a TOUR_SECTION holds exactly n ids before its -1
+            if (!tsplib && ids.size() == n) {
+                ended = true;
+                break;
+            }
             uint64_t v;
             if (!parse_u64(t, &v) || ids.size() >= n) return LOGO2OPT_EFORMAT;
             if (v < 1 || v > n) return LOGO2OPT_ETOUR;

@@ -275,7 +275,9 @@ def test_tour_write_read_round_trip(tmp_path):
     ("1 2 3 4\n", 0),
     ("TYPE : TSP\nTOUR_SECTION\n1\n2\n3\n4\n-1\n", -9),
     ("1\n2\n3\n", -9),
-    ("1\n2\n3\n4\n5\n", -9),
+    ("1\n2\n3\n4\n5\n", 0),  # header-less: the first n ids (routeInitFromFile reads n values)
+    ("1\n2\n3\n4\n-1\n", 0),
+    ("TYPE : TOUR\nTOUR_SECTION\n1\n2\n3\n4\n5\n-1\n", -9),  # a TOUR_SECTION holds exactly n ids
     ("1\n2\n2\n4\n", -3),
     ("0\n1\n2\n3\n", -3),
     ("1\n2\nx\n4\n", -9),
