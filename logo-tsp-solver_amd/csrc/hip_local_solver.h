@@ -92,7 +92,10 @@ public:
 
     // row pruning of the sweep (no counterpart in the reference): same moves,
     // faster local searches on large instances; see logo2opt_set_pruning
-    void setPruning(bool on) { check(logo2opt_set_pruning(ctx_, on ? 1 : 0), "logo2opt_set_pruning"); }
+    void setPruning(bool on)
+    {
+        check(logo2opt_set_pruning(ctx_, on ? LOGO2OPT_PRUNE_ALL : LOGO2OPT_PRUNE_OFF), "logo2opt_set_pruning");
+    }
 
     uint64_t routeLength(const std::vector<uint32_t> &route)
     {

@@ -223,6 +223,7 @@ struct logo2opt_engine {
     struct group {
         sweep_cfg cfg;
         uint32_t off, count, id;
+        bool prune;  // launched as the row-pruning variant
     };
     std::vector<group> groups;
     l2o_sweep_item *d_sitems = nullptr;
@@ -251,8 +252,10 @@ struct logo2opt_engine {
     // (logo2opt_engine_set_graphs, or LOGO2OPT_GRAPHS=0 in the environment)
     bool graphs = true;
     // sweeps skip the blocks of rows that the bounding-box bound rules out
-    // (logo2opt_engine_set_pruning; same moves, fewer candidates evaluated)
-    bool prune = false;
+    // (logo2opt_engine_set_pruning; same moves, fewer candidates evaluated):
+    // LOGO2OPT_PRUNE_OFF, _ALL, or _AUTO (instances of L2O_PRUNE_AUTO_MIN_N
+    // cities or more)
+    int prune = LOGO2OPT_PRUNE_OFF;
     std::vector<hipEvent_t> tev;
     uint32_t tev_used = 0;
 };
@@ -263,7 +266,7 @@ struct logo2opt_ctx {
     uint32_t ge_chains = 0;
     uint32_t n = 0;
     uint32_t metric = LOGO2OPT_METRIC_EUC_2D;
-    bool prune = false;  // applied to ge when it is created
+    int prune = LOGO2OPT_PRUNE_OFF;  // applied to ge when it is created
     std::vector<logo2opt_city_coords> coords;
     int32_t slot = -1;
     uint64_t sweeps = 0, moves = 0, evals = 0, fallbacks = 0;
@@ -399,6 +402,11 @@ void drop_graph(logo2opt_engine *e)
     e->gsteps = 0;
 }
 
+// LOGO2OPT_PRUNE_AUTO prunes instances of at least this many cities: below
+// it the pruned sweep measured 4-9% slower (per-wave fixed costs dominate),
+// from 16,000 on it wins (1.04x there, 1.2x at 24,000, 1.6x at 33,810)
+#define L2O_PRUNE_AUTO_MIN_N 16000u
+
 // 1/L2O_TAIL_SPLIT of a launch's work, at its end, goes in items of
 // 1/L2O_TAIL_DIV of the shape's rows (T/L2O_TAIL_DIV must stay a multiple of 4)
 #ifndef L2O_TAIL_SPLIT
@@ -426,7 +434,7 @@ uint64_t item_work(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
 
 int rebuild_lists(logo2opt_engine *e)
 {
-    std::map<std::tuple<int, int, int>, std::vector<l2o_sweep_item>> by_cfg;
+    std::map<std::tuple<int, int, int, bool>, std::vector<l2o_sweep_item>> by_cfg;
     std::vector<l2o_apply_item> ai;
     e->ils_grid = 0;
     e->any_ils = false;
@@ -443,7 +451,8 @@ int rebuild_lists(logo2opt_engine *e)
         const slot_t &s = e->slots[id];
         const uint32_t n = s.n;
         const sweep_cfg c = cfg_for(n, table);
-        auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves)];
+        const bool pr = e->prune == LOGO2OPT_PRUNE_ALL || (e->prune == LOGO2OPT_PRUNE_AUTO && n >= L2O_PRUNE_AUTO_MIN_N);
+        auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves, pr)];
         // rows a <= n-4; the first tile of a row block starts at the
         // diagonal, b0 = a0 + 2
         for (uint32_t a0 = 0; a0 + 4 <= n; a0 += c.t)
@@ -497,7 +506,7 @@ int rebuild_lists(logo2opt_engine *e)
         v.swap(sorted);
         const uint32_t cnt = (uint32_t)v.size();
         if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;
-        e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size()});
+        e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size(), std::get<3>(kv.first)});
         all.insert(all.end(), v.begin(), v.end());
     }
     if (all.size() > e->sitems_cap) {
@@ -538,7 +547,7 @@ int issue_step(logo2opt_engine *e)
         L2O_TRY(hipEventRecord(e->tev[e->tev_used], e->stream));
     }
     for (const auto &g : e->groups) {
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune));
         e->sweep_launches += 1;
     }
     if (tm) {
@@ -1070,20 +1079,21 @@ int logo2opt_engine_set_graphs(logo2opt_engine *e, int on)
     return LOGO2OPT_OK;
 }
 
-int logo2opt_engine_set_pruning(logo2opt_engine *e, int on)
+int logo2opt_engine_set_pruning(logo2opt_engine *e, int mode)
 {
-    if (!e) return LOGO2OPT_EINVAL;
-    e->prune = on != 0;
+    if (!e || mode < LOGO2OPT_PRUNE_OFF || mode > LOGO2OPT_PRUNE_AUTO) return LOGO2OPT_EINVAL;
+    e->prune = mode;
+    e->dirty = true;  // the work lists group launches by it
     drop_graph(e);
     return LOGO2OPT_OK;
 }
 
-int logo2opt_set_pruning(logo2opt_ctx *ctx, int on)
+int logo2opt_set_pruning(logo2opt_ctx *ctx, int mode)
 {
     if (!ctx) return LOGO2OPT_EINVAL;
-    int rc = logo2opt_engine_set_pruning(ctx->e, on);
-    if (!rc && ctx->ge) rc = logo2opt_engine_set_pruning(ctx->ge, on);
-    if (!rc) ctx->prune = on != 0;
+    int rc = logo2opt_engine_set_pruning(ctx->e, mode);
+    if (!rc && ctx->ge) rc = logo2opt_engine_set_pruning(ctx->ge, mode);
+    if (!rc) ctx->prune = mode;
     return rc;
 }
 
@@ -1173,7 +1183,7 @@ int logo2opt_step(logo2opt_ctx *ctx, const uint32_t *route, logo2opt_best2_out *
     const double h1 = now_ms();
     L2O_TRY(hipEventRecord(e->ev0, e->stream));
     for (const auto &g : e->groups)
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune));
     L2O_TRY(hipEventRecord(e->ev1, e->stream));
     L2O_TRY(hipEventSynchronize(e->ev1));
     float ms = 0.0f;
@@ -1366,11 +1376,11 @@ int logo2opt_engine_time_sweep(logo2opt_engine *e, uint32_t sweeps, double *avg_
     int rc = rebuild_lists(e);
     if (rc) return rc;
     for (const auto &g : e->groups)  // untimed first launch
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune));
     L2O_TRY(hipEventRecord(e->ev0, e->stream));
     for (uint32_t s = 0; s < sweeps; ++s)
         for (const auto &g : e->groups)
-            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
+            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune));
     L2O_TRY(hipEventRecord(e->ev1, e->stream));
     L2O_TRY(hipEventSynchronize(e->ev1));
     float ms = 0.0f;
@@ -1407,11 +1417,11 @@ int logo2opt_time_sweep(logo2opt_ctx *ctx, const uint32_t *route, uint32_t sweep
     if (rc) return rc;
     // one untimed launch (code object load, caches)
     for (const auto &g : e->groups)
-        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
+        L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune));
     L2O_TRY(hipEventRecord(e->ev0, e->stream));
     for (uint32_t s = 0; s < sweeps; ++s)
         for (const auto &g : e->groups)
-            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, e->prune));
+            L2O_TRY(l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune));
     L2O_TRY(hipEventRecord(e->ev1, e->stream));
     L2O_TRY(hipEventSynchronize(e->ev1));
     float ms = 0.0f;

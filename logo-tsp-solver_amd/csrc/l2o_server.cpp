@@ -229,10 +229,10 @@ int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_
     return LOGO2OPT_OK;
 }
 
-int logo2opt_server_set_pruning(logo2opt_server *s, int on)
+int logo2opt_server_set_pruning(logo2opt_server *s, int mode)
 {
     if (!s) return LOGO2OPT_EINVAL;
-    return logo2opt_engine_set_pruning(s->e, on);
+    return logo2opt_engine_set_pruning(s->e, mode);
 }
 
 int logo2opt_server_counts(logo2opt_server *s, uint64_t *counts)

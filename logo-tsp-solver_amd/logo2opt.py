@@ -19,6 +19,12 @@ METRIC_EUC_2D = 0
 METRIC_CEIL_2D = 1
 METRIC_ATT = 2
 METRICS = {"EUC_2D": METRIC_EUC_2D, "CEIL_2D": METRIC_CEIL_2D, "ATT": METRIC_ATT}
+PRUNE_OFF, PRUNE_ALL, PRUNE_AUTO = 0, 1, 2
+
+
+def _prune_mode(on):
+    """True / False map to PRUNE_ALL / PRUNE_OFF; ints pass through."""
+    return int(on) if not isinstance(on, bool) else (PRUNE_ALL if on else PRUNE_OFF)
 TSP_ROUND = 1
 EFORMAT = -9
 EIO = -10
@@ -374,7 +380,7 @@ class Context:
 
     def set_pruning(self, on):
         """Skip blocks of rows the bounding-box bound rules out (same results)."""
-        _check(load().logo2opt_set_pruning(self.h, 1 if on else 0), "logo2opt_set_pruning")
+        _check(load().logo2opt_set_pruning(self.h, _prune_mode(on)), "logo2opt_set_pruning")
 
     def time_sweep(self, route, sweeps):
         ms = C.c_double()
@@ -492,7 +498,7 @@ class Engine:
         _check(load().logo2opt_engine_set_graphs(self.h, 1 if on else 0), "logo2opt_engine_set_graphs")
 
     def set_pruning(self, on):
-        _check(load().logo2opt_engine_set_pruning(self.h, 1 if on else 0), "logo2opt_engine_set_pruning")
+        _check(load().logo2opt_engine_set_pruning(self.h, _prune_mode(on)), "logo2opt_engine_set_pruning")
 
     def stats(self):
         s = Stats()
@@ -584,7 +590,7 @@ class Server:
         _check(load().logo2opt_server_set_limits(self.h, max_running, max_inflight_pairs), "server_set_limits")
 
     def set_pruning(self, on):
-        _check(load().logo2opt_server_set_pruning(self.h, 1 if on else 0), "server_set_pruning")
+        _check(load().logo2opt_server_set_pruning(self.h, _prune_mode(on)), "server_set_pruning")
 
     def counts(self):
         c = np.zeros(6, np.uint64)

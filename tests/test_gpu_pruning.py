@@ -107,3 +107,26 @@ def test_pruning_toggles_between_runs_of_one_engine():
             res = e.result(s, len(r0))
             assert res["length"] == Lo and res["route"].tolist() == r.tolist(), prune
             e.remove(s)
+
+
+def test_auto_mode_prunes_only_large_instances_and_changes_no_result():
+    """LOGO2OPT_PRUNE_AUTO launches the pruned sweep for instances of 16,000
+    cities or more and the full one below, in the same engine step; every
+    result equals the unpruned engine's."""
+    sizes = [(17000, True), (3000, False), (700, False)]
+    inst = []
+    for k, (n, clustered) in enumerate(sizes):
+        xy = O.gen_clustered(n, 10**6, 40 + k) if clustered else O.gen_uniform(n, 10**6, 40 + k)
+        inst.append((n, L.coords_of(xy), O.nn_tour(xy, 0)))
+    results = {}
+    for mode in (L.PRUNE_OFF, L.PRUNE_AUTO, L.PRUNE_ALL):
+        with L.Engine(hbm_budget=1 << 28, max_slots=8) as e:
+            e.set_pruning(mode)
+            slots = [e.add(c, r0, ils_rounds=2, seed=5) for n, c, r0 in inst]
+            assert e.run(sync_every=16) == 0
+            results[mode] = [(e.result(s, n)["length"], e.result(s, n)["route"].tolist())
+                             for s, (n, _, _) in zip(slots, inst)]
+    assert results[L.PRUNE_AUTO] == results[L.PRUNE_OFF] == results[L.PRUNE_ALL]
+    with L.Engine(hbm_budget=1 << 20, max_slots=1) as e:
+        with pytest.raises(L.Logo2OptError):
+            e.set_pruning(3)
