@@ -21,6 +21,10 @@ Also reported on the same line:
                 engine (batched sweeps, graph-replayed steps) versus the plain
                 one-instance-per-launch path (host round trip per move) on an
                 every-16th subset; final lengths cross-checked between the two
+  pruned, batched.pruned, serving_pruned
+                the same local search, batch and serving replay with the sweep's
+                row pruning on: separate figures (fewer candidates evaluated,
+                identical results, asserted), never the headline
 """
 import argparse
 import ctypes as C
@@ -148,7 +152,7 @@ def cpu_baseline(n, xy, r0, budget_s):
             "sample": f"{sweeps} exact oracle sweeps ({el:.1f} s) of the 33,810-city clustered instance"}
 
 
-def batched_vs_plain(count, plain_every, seed=4):
+def batched_vs_plain(count, plain_every, seed=4, prune=True):
     rs = np.random.default_rng(seed)
     sizes = np.rint(np.exp(rs.uniform(np.log(100), np.log(5000), count))).astype(int)
     insts = []
@@ -165,6 +169,17 @@ def batched_vs_plain(count, plain_every, seed=4):
         assert live == 0
         st = eng.stats()
         lengths = [eng.result(s)["length"] for s in slots]
+    tbp = None
+    if prune:
+        # the same batch with row pruning on every instance: a separate
+        # figure (fewer candidates evaluated), lengths asserted equal
+        with L.Engine(hbm_budget=budget, max_slots=count) as eng:
+            eng.set_pruning(L.PRUNE_ALL)
+            slots = [eng.add(c, r) for n, c, r in insts]
+            t0 = time.perf_counter()
+            assert eng.run(sync_every=32) == 0
+            tbp = time.perf_counter() - t0
+            assert [eng.result(s)["length"] for s in slots] == lengths
     subset = list(range(0, count, plain_every))
     ctxs = [(k, L.Context(insts[k][1])) for k in subset]
     tp = 0.0
@@ -177,13 +192,15 @@ def batched_vs_plain(count, plain_every, seed=4):
         ctx.close()
     ib = count / tb
     ip = len(subset) / tp
+    extra = {} if tbp is None else {"pruned": {"batched_s": tbp, "batched_instances_per_s": count / tbp,
+                                               "speedup_vs_plain": count / tbp / ip, "same_lengths": True}}
     return {
         "workload": f"{count} instances, n log-uniform in [100,5000] (seed {seed}), coords U{{0..9999}}^2, "
                     "random starts, each to a 2-opt local optimum",
         "instances": count, "batched_s": tb, "batched_instances_per_s": ib,
         "batched_evals_per_s": st["evals"] / tb, "moves": st["moves"], "sweeps": st["sweeps"],
         "plain_sample": len(subset), "plain_s": tp, "plain_instances_per_s": ip,
-        "speedup": ib / ip, "plain_vs_batched_length_mismatches": int(mism),
+        "speedup": ib / ip, "plain_vs_batched_length_mismatches": int(mism), **extra,
     }
 
 
@@ -433,7 +450,7 @@ def main():
         res["cpu_baseline"] = cpu_baseline(n, xy, r0, args.cpu_budget)
     if not args.skip_batch:
         log("batched vs plain")
-        res["batched"] = batched_vs_plain(args.batch_count, args.plain_every)
+        res["batched"] = batched_vs_plain(args.batch_count, args.plain_every, prune=not args.skip_pruned)
     if not args.skip_serve:
         log("serving")
         rates = [float(v) for v in args.serve_rates.split(",")]
