@@ -245,9 +245,10 @@ int logo2opt_engine_set_graphs(logo2opt_engine *e, int on);
  * identical; fewer candidates are evaluated, so the sweep's evals/s figure
  * does not apply.  mode: LOGO2OPT_PRUNE_OFF (the default), LOGO2OPT_PRUNE_ALL
  * (every instance), or LOGO2OPT_PRUNE_AUTO (instances of 16,000 cities or
- * more, where it is faster: below that the pruned sweep measured 4-9% slower
- * on MI355X).  No counterpart in the reference (its kernels evaluate every
- * pair, opt_kernel.cu:34-262). */
+ * more).  On MI355X a lone instance's local search gains from about 16,000
+ * cities (4-9% slower below: AUTO); batches of many instances gain at every
+ * size measured (ALL).  No counterpart in the reference (its kernels evaluate
+ * every pair, opt_kernel.cu:34-262). */
 #define LOGO2OPT_PRUNE_OFF  0
 #define LOGO2OPT_PRUNE_ALL  1
 #define LOGO2OPT_PRUNE_AUTO 2
