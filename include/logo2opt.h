@@ -96,9 +96,12 @@ int logo2opt_destroy(logo2opt_ctx *ctx);
 
 /* CUDASolver::optimizeStep (CUDASolver.cpp:285-386): one best-improvement
  * sweep of `route`, no apply.  out->minchange = 0 (and i = j = 0) when no
- * improving move exists; otherwise out->i < out->j.  t (optional) receives the
- * HtD / kernel / DtH times in seconds like CUDASolver::benchmark
- * (CUDASolver.cpp:146-283), measured with HIP events. */
+ * improving move exists; otherwise out->i < out->j.  t (optional) receives
+ * times in seconds like CUDASolver::benchmark (CUDASolver.cpp:146-283):
+ * kernel_time is the sweep kernel's duration from HIP events; HtD_time is the
+ * host wall time of building the tour-ordered points (with their exact edge
+ * lengths) and uploading them and the work list, synchronously; DtH_time the
+ * host wall time of reading the result back. */
 int logo2opt_step(logo2opt_ctx *ctx, const uint32_t *route, logo2opt_best2_out *out,
                   logo2opt_process_time *t);
 
@@ -106,9 +109,11 @@ int logo2opt_step(logo2opt_ctx *ctx, const uint32_t *route, logo2opt_best2_out *
  * device: sweep, apply on the device, repeat to a local optimum.
  * route_inout: start tour in, local optimum out.  length (optional): final
  * uint64 length.  moves (optional): applied moves.  log (optional): the first
- * log_cap applied moves (i, J, gain) in order.  sync_every = 1 reproduces the
- * reference's one-round-trip-per-move loop; sync_every = K > 1 replays K
- * sweep+apply steps as one captured HIP graph between host checks. */
+ * min(moves, log_cap) applied moves (i, J, gain) in order, log_cap < 2^32
+ * (LOGO2OPT_EINVAL above); the context grows its device buffers to fit.
+ * sync_every = 1 reproduces the reference's one-round-trip-per-move loop;
+ * sync_every = K > 1 replays K sweep+apply steps as one captured HIP graph
+ * between host checks. */
 int logo2opt_optimize(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t *length,
                       uint64_t *moves, logo2opt_best2_out *log, uint64_t log_cap,
                       uint32_t sync_every);
@@ -201,7 +206,9 @@ int logo2opt_engine_destroy(logo2opt_engine *e);
 uint64_t logo2opt_engine_bytes_for(uint32_t n, const logo2opt_job *job);
 /* Admit an instance: copies coordinates and start tour to the device.
  * Returns LOGO2OPT_EAGAIN (nothing changed) when the HBM budget or the slot
- * count is exhausted.  The new slot joins the next batched step. */
+ * count is exhausted for now, and LOGO2OPT_ENOMEM when the instance needs
+ * more than the whole budget (logo2opt_engine_bytes_for), so it can never be
+ * admitted.  The new slot joins the next batched step. */
 int logo2opt_engine_add(logo2opt_engine *e, uint32_t n, const logo2opt_city_coords *coords,
                         const uint32_t *route, const logo2opt_job *job, int32_t *slot);
 /* Run up to max_steps batched steps (each: one sweep of every active
@@ -266,16 +273,24 @@ typedef struct logo2opt_server logo2opt_server;
 int logo2opt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots, uint32_t queue_cap,
                            logo2opt_server **out);
 int logo2opt_server_destroy(logo2opt_server *s);
-/* Queue a request (copied).  LOGO2OPT_EAGAIN (id = -1) when the queue is full. */
+/* Queue a request (copied).  LOGO2OPT_EAGAIN (id = -1) when the queue is full.
+ * route may be NULL: the server then builds the greedy nearest-neighbour tour
+ * from city 0 (logo2opt_nn_tour) when it first tries to admit the request;
+ * that time is part of the request's latency. */
 int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_coords *coords,
                            const uint32_t *route, const logo2opt_job *job, int64_t *id);
 /* Admit what fits, run up to `steps` batched steps (host check every
  * sync_every), harvest finished requests. */
 int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every, uint32_t *running,
                          uint32_t *queued);
-/* Ids completed since the last poll. */
+/* Ids completed since the last poll, including requests rejected at
+ * admission (their result reports the error). */
 int logo2opt_server_poll(logo2opt_server *s, int64_t *ids, uint32_t cap, uint32_t *count);
-/* Result of a completed request; latency_s = submit -> done, queue_s = submit -> admission. */
+/* Result of a completed request; latency_s = submit -> done, queue_s = submit -> admission.
+ * A request rejected at admission returns its error code here (for example
+ * LOGO2OPT_ENOMEM: it needs more than the server's whole HBM budget, or
+ * LOGO2OPT_ECOORD / LOGO2OPT_ETOUR for invalid input), with latency_s set;
+ * LOGO2OPT_ESTATE while a request is queued or running. */
 int logo2opt_server_result(logo2opt_server *s, int64_t id, uint32_t *route_out, logo2opt_result *res,
                            double *latency_s, double *queue_s);
 /* Forget a request (cancels it if queued or running). */

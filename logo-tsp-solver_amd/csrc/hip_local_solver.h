@@ -62,15 +62,22 @@ public:
         return out;
     }
 
-    // local search to a 2-opt local optimum; bestLength receives the final
-    // length when it improves on its last entry (the reference only reads
-    // bestLength for its progress trace, twoOptLocalSolver.cpp:52-61)
+    // local search to a 2-opt local optimum.  bestLength is left untouched:
+    // TwoOptLocalSolver::optimize only reads it for its progress trace
+    // (twoOptLocalSolver.cpp:52-61), and the ILS drivers that pass their
+    // global best list compare the new tour's length against its last entry
+    // afterwards (ILSGlobalSolverSequential.cpp:74-92,
+    // ILSGlobalSolverMT.cpp:137-178): a length pushed here would make every
+    // improvement look like a tie and be reverted.  lastLength() and
+    // lastMoves() give the result of the last call.
     void optimize(std::vector<uint32_t> &route, std::vector<unsigned long> &bestLength)
     {
-        uint64_t length = 0, moves = 0;
-        check(logo2opt_optimize(ctx_, route.data(), &length, &moves, nullptr, 0, sync_every_), "logo2opt_optimize");
-        if (bestLength.empty() || length < bestLength.back()) bestLength.push_back((unsigned long)length);
+        (void)bestLength;
+        check(logo2opt_optimize(ctx_, route.data(), &last_length_, &last_moves_, nullptr, 0, sync_every_),
+              "logo2opt_optimize");
     }
+    uint64_t lastLength() const { return last_length_; }
+    uint64_t lastMoves() const { return last_moves_; }
 
     // ILS on the device (ILSGlobalSolverSequential.cpp:60-100 with a fixed
     // round count and the seeded double-bridge kick)
@@ -116,6 +123,7 @@ private:
     uint32_t sync_every_;
     int metric_;
     logo2opt_ctx *ctx_ = nullptr;
+    uint64_t last_length_ = 0, last_moves_ = 0;
 };
 
 namespace detail {

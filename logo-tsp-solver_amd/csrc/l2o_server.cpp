@@ -42,8 +42,33 @@ struct request {
     double t_submit = 0, t_admit = 0, t_done = 0;
     int32_t slot = -1;
     int state = Q_QUEUED;
+    int err = LOGO2OPT_OK;  // Q_REJECTED: why
+    bool construct = false;  // no start tour given: the server builds the NN tour
+    double t_construct = 0;  // seconds spent building it
     logo2opt_result res{};
 };
+
+// Greedy nearest-neighbour tour from city 0 (the serving workload's start)
+// over the request's coordinates, translated so the minimum is 0.  Invalid
+// coordinates are left to engine_add to reject.
+int construct_nn(request &r)
+{
+    const uint32_t n = r.n;
+    double mnx = r.coords[0].x, mny = r.coords[0].y;
+    for (uint32_t k = 1; k < n; ++k) {
+        mnx = std::min(mnx, (double)r.coords[k].x);
+        mny = std::min(mny, (double)r.coords[k].y);
+    }
+    std::vector<int32_t> xy(2 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const double x = (double)r.coords[k].x - mnx, y = (double)r.coords[k].y - mny;
+        if (!(x >= 0 && x < 2147483648.0 && y >= 0 && y < 2147483648.0)) return LOGO2OPT_ECOORD;
+        xy[2 * k] = (int32_t)x;
+        xy[2 * k + 1] = (int32_t)y;
+    }
+    r.route.resize(n);
+    return logo2opt_nn_tour(n, xy.data(), 0, r.route.data());
+}
 
 }  // namespace
 
@@ -94,7 +119,7 @@ int logo2opt_server_destroy(logo2opt_server *s)
 int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_coords *coords, const uint32_t *route,
                            const logo2opt_job *job, int64_t *id)
 {
-    if (!s || !coords || !route || !id || n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
+    if (!s || !coords || !id || n < 3 || n > LOGO2OPT_MAX_N) return LOGO2OPT_EINVAL;
     s->n_submitted++;
     if (s->queue.size() >= s->queue_cap) {
         s->n_rejected++;
@@ -105,7 +130,8 @@ int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_c
     r.id = s->next_id++;
     r.n = n;
     r.coords.assign(coords, coords + n);
-    r.route.assign(route, route + n);
+    if (route) r.route.assign(route, route + n);
+    else r.construct = true;
     if (job) r.job = *job;
     r.t_submit = now_s();
     *id = r.id;
@@ -126,13 +152,24 @@ int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every
         if (s->max_inflight_pairs && !s->slot_req.empty() && s->inflight_pairs + pairs > s->max_inflight_pairs)
             break;
         int32_t slot = -1;
-        const int rc = logo2opt_engine_add(s->e, r.n, r.coords.data(), r.route.data(), &r.job, &slot);
+        int rc = LOGO2OPT_OK;
+        if (r.construct) {  // once, at the first admission attempt
+            const double t0 = now_s();
+            rc = construct_nn(r);
+            r.t_construct = now_s() - t0;
+            r.construct = false;
+        }
+        if (rc == LOGO2OPT_OK) rc = logo2opt_engine_add(s->e, r.n, r.coords.data(), r.route.data(), &r.job, &slot);
         if (rc == LOGO2OPT_EAGAIN) break;  // head of line does not fit yet
         s->queue.pop_front();
-        if (rc) {  // invalid request: finished with an error
+        if (rc) {  // invalid, or larger than the whole budget: finished with an error
             r.state = Q_REJECTED;
+            r.err = rc;
             r.t_done = now_s();
             s->n_rejected++;
+            s->completed.push_back(r.id);
+            std::vector<logo2opt_city_coords>().swap(r.coords);
+            std::vector<uint32_t>().swap(r.route);
             continue;
         }
         r.slot = slot;
@@ -195,6 +232,11 @@ int logo2opt_server_result(logo2opt_server *s, int64_t id, uint32_t *route_out, 
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return LOGO2OPT_ENOTFOUND;
     request &r = it->second;
+    if (r.state == Q_REJECTED) {
+        if (latency_s) *latency_s = r.t_done - r.t_submit;
+        if (queue_s) *queue_s = -1.0;
+        return r.err;
+    }
     if (r.state != Q_DONE) return LOGO2OPT_ESTATE;
     if (route_out) std::memcpy(route_out, r.route.data(), 4ull * r.n);
     if (res) *res = r.res;
@@ -295,6 +337,10 @@ int logo2opt_server_replay(logo2opt_server *s, uint32_t count, const double *arr
                 auto it = idx.find(done[k]);
                 if (it == idx.end()) continue;
                 const request &r = s->reqs[done[k]];
+                if (r.state == Q_REJECTED) {  // never admitted: latency stays -1
+                    ++finished;
+                    continue;
+                }
                 latency_out[it->second] = (r.t_done - t0) - arrival_s[it->second];
                 if (queue_out) queue_out[it->second] = (r.t_admit - t0) - arrival_s[it->second];
                 ++finished;

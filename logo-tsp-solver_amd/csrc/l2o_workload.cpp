@@ -12,6 +12,7 @@
 #include <functional>
 #include <numeric>
 #include <queue>
+#include <thread>
 #include <vector>
 
 #include "../../include/logo2opt.h"
@@ -134,6 +135,89 @@ struct point_grid {
         *s_out = best;
         return arg;
     }
+
+    // The k points u != c that come first in (squared distance, u) order,
+    // ascending, into ids / ss; returns how many (fewer than k only when the
+    // grid holds fewer points).  Same ring bound as nearest(): once the k-th
+    // best distance is below the ring's lower bound, no later point can enter.
+    int knn(uint32_t c, int k, uint32_t *ids, uint64_t *ss) const
+    {
+        const int64_t px = xy[2 * c], py = xy[2 * c + 1];
+        const int64_t gx = (px - mnx) / cw, gy = (py - mny) / cw;
+        int cnt = 0;
+        for (int64_t R = 0; R < G; ++R) {
+            if (R >= 2 && cnt == k) {
+                const uint64_t lb = (uint64_t)((R - 1) * cw) * (uint64_t)((R - 1) * cw);
+                if (lb > ss[k - 1]) break;
+            }
+            const int64_t y0 = std::max<int64_t>(0, gy - R), y1 = std::min<int64_t>(G - 1, gy + R);
+            for (int64_t y = y0; y <= y1; ++y) {
+                const bool full = (y == gy - R || y == gy + R);
+                const int64_t step = full ? 1 : 2 * R;
+                for (int64_t x = gx - R; x <= gx + R; x += (step > 0 ? step : 1)) {
+                    if (x < 0 || x >= G) continue;
+                    for (uint32_t u : cells[(size_t)(y * G + x)]) {
+                        if (u == c) continue;
+                        const int64_t dx = xy[2 * u] - px, dy = xy[2 * u + 1] - py;
+                        const uint64_t s = (uint64_t)(dx * dx + dy * dy);
+                        if (cnt == k && (s > ss[k - 1] || (s == ss[k - 1] && u > ids[k - 1]))) continue;
+                        int q = cnt < k ? cnt++ : k - 1;  // insertion into the sorted list
+                        while (q > 0 && (ss[q - 1] > s || (ss[q - 1] == s && ids[q - 1] > u))) {
+                            ss[q] = ss[q - 1];
+                            ids[q] = ids[q - 1];
+                            --q;
+                        }
+                        ss[q] = s;
+                        ids[q] = u;
+                    }
+                }
+            }
+        }
+        return cnt;
+    }
+};
+
+// Each city's K nearest other cities in (squared distance, id) order,
+// computed on several host threads (the queries only read the grid).  The
+// greedy constructions below take their next city from these lists: the
+// first entry that is still allowed is the allowed city of smallest
+// (distance, id), because every city outside the list comes later in that
+// order than the whole list; only when no entry is allowed any more do they
+// search the grid.
+constexpr int L2O_KNN = 10;
+
+struct knn_lists {
+    std::vector<uint32_t> ids;  // n x L2O_KNN
+    std::vector<uint8_t> cnt;
+
+    void build(const point_grid &g, uint32_t n)
+    {
+        ids.assign((size_t)n * L2O_KNN, UINT32_MAX);
+        cnt.assign(n, 0);
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const uint32_t threads = std::min<uint32_t>(std::min(16u, hw), n / 2048 + 1);
+        auto work = [&](uint32_t lo, uint32_t hi) {
+            uint64_t ss[L2O_KNN];
+            for (uint32_t c = lo; c < hi; ++c) cnt[c] = (uint8_t)g.knn(c, L2O_KNN, &ids[(size_t)c * L2O_KNN], ss);
+        };
+        if (threads <= 1) {
+            work(0, n);
+            return;
+        }
+        std::vector<std::thread> pool;
+        for (uint32_t t = 0; t < threads; ++t)
+            pool.emplace_back(work, (uint32_t)((uint64_t)n * t / threads), (uint32_t)((uint64_t)n * (t + 1) / threads));
+        for (auto &th : pool) th.join();
+    }
+    // the first city u of c's list with ok(u), or UINT32_MAX
+    template <class F>
+    uint32_t first(uint32_t c, F ok) const
+    {
+        const uint32_t *l = &ids[(size_t)c * L2O_KNN];
+        for (int q = 0; q < cnt[c]; ++q)
+            if (ok(l[q])) return l[q];
+        return UINT32_MAX;
+    }
 };
 
 }  // namespace
@@ -197,80 +281,34 @@ void logo2opt_xy_to_coords(uint32_t n, const int32_t *xy, logo2opt_city_coords *
     }
 }
 
-// Greedy nearest neighbour on a uniform bucket grid.  From the current city
-// the rings of cells around it are searched outward; ring R cannot hold a
-// point closer than (R-1) cell widths, so the search stops once that bound
-// exceeds the best squared distance found (ties: lowest city id, so equal
-// distances in farther rings are still examined).
+// Greedy nearest neighbour: from the current city the next one is the
+// unvisited city of smallest (squared distance, id) -- lowest city id on
+// ties.  It comes from the current city's nearest-city list when one of its
+// entries is unvisited; otherwise from the grid of unvisited cities, whose
+// rings of cells are searched outward until the ring's lower bound exceeds the
+// best distance found (equal distances in farther rings are still examined).
 int logo2opt_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route)
 {
     if (n == 0 || !xy || !route || start >= n) return LOGO2OPT_EINVAL;
-    int64_t mnx = xy[0], mny = xy[1], mxx = xy[0], mxy = xy[1];
-    for (uint32_t k = 1; k < n; ++k) {
-        mnx = std::min<int64_t>(mnx, xy[2 * k]);
-        mxx = std::max<int64_t>(mxx, xy[2 * k]);
-        mny = std::min<int64_t>(mny, xy[2 * k + 1]);
-        mxy = std::max<int64_t>(mxy, xy[2 * k + 1]);
-    }
-    const int64_t span = std::max<int64_t>(1, std::max(mxx - mnx, mxy - mny) + 1);
-    int64_t G = (int64_t)std::ceil(std::sqrt((double)n / 2.0));
-    G = std::max<int64_t>(1, std::min<int64_t>(G, 4096));
-    const int64_t cw = (span + G - 1) / G;  // cell width, G * cw >= span
-    auto cell_of = [&](uint32_t c, int64_t *gx, int64_t *gy) {
-        *gx = (xy[2 * c] - mnx) / cw;
-        *gy = (xy[2 * c + 1] - mny) / cw;
-    };
-    std::vector<std::vector<uint32_t>> cells((size_t)(G * G));
-    std::vector<uint32_t> pos(n);
-    for (uint32_t c = 0; c < n; ++c) {
-        int64_t gx, gy;
-        cell_of(c, &gx, &gy);
-        auto &v = cells[(size_t)(gy * G + gx)];
-        pos[c] = (uint32_t)v.size();
-        v.push_back(c);
-    }
-    auto remove = [&](uint32_t c) {
-        int64_t gx, gy;
-        cell_of(c, &gx, &gy);
-        auto &v = cells[(size_t)(gy * G + gx)];
-        const uint32_t p = pos[c], last = v.back();
-        v[p] = last;
-        pos[last] = p;
-        v.pop_back();
-    };
+    point_grid g;
+    g.build(n, xy, 2.0);
+    knn_lists nl;
+    nl.build(g, n);
+    std::vector<uint8_t> visited(n, 0);
     uint32_t cur = start;
-    remove(cur);
+    visited[cur] = 1;
+    g.remove(cur);
     route[0] = cur;
     for (uint32_t k = 1; k < n; ++k) {
-        int64_t gx, gy;
-        cell_of(cur, &gx, &gy);
-        const int64_t px = xy[2 * cur], py = xy[2 * cur + 1];
-        uint64_t best = UINT64_MAX;
-        uint32_t arg = UINT32_MAX;
-        for (int64_t R = 0; R < G; ++R) {
-            if (R >= 2 && best != UINT64_MAX) {
-                const uint64_t lb = (uint64_t)((R - 1) * cw) * (uint64_t)((R - 1) * cw);
-                if (lb > best) break;
-            }
-            const int64_t x0 = gx - R, x1 = gx + R, y0 = gy - R, y1 = gy + R;
-            for (int64_t y = std::max<int64_t>(0, y0); y <= std::min<int64_t>(G - 1, y1); ++y) {
-                const bool edge_row = (y == y0 || y == y1);
-                for (int64_t x = std::max<int64_t>(0, x0); x <= std::min<int64_t>(G - 1, x1); ++x) {
-                    if (!edge_row && x != x0 && x != x1) continue;
-                    for (uint32_t u : cells[(size_t)(y * G + x)]) {
-                        const int64_t dx = xy[2 * u] - px, dy = xy[2 * u + 1] - py;
-                        const uint64_t s = (uint64_t)(dx * dx + dy * dy);
-                        if (s < best || (s == best && u < arg)) {
-                            best = s;
-                            arg = u;
-                        }
-                    }
-                }
-            }
+        uint32_t u = nl.first(cur, [&](uint32_t v) { return !visited[v]; });
+        if (u == UINT32_MAX) {
+            uint64_t s;
+            u = g.nearest(cur, [](uint32_t) { return true; }, &s);  // the grid holds the unvisited cities
         }
-        remove(arg);
-        route[k] = arg;
-        cur = arg;
+        visited[u] = 1;
+        g.remove(u);
+        route[k] = u;
+        cur = u;
     }
     return LOGO2OPT_OK;
 }
@@ -316,14 +354,34 @@ int logo2opt_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route)
             return x > o.x;
         }
     };
+    knn_lists nl;
+    nl.build(g, n);
+    auto sq = [&](uint32_t a, uint32_t b) {
+        const int64_t dx = (int64_t)xy[2 * a] - xy[2 * b], dy = (int64_t)xy[2 * a + 1] - xy[2 * b + 1];
+        return (uint64_t)(dx * dx + dy * dy);
+    };
     std::priority_queue<cand, std::vector<cand>, std::greater<cand>> pq;
+    // x's nearest valid partner: degree < 2 (the grid holds only those) and
+    // in another fragment
     auto query = [&](uint32_t x) {
         const uint32_t rx = find(x);
+        uint32_t y = nl.first(x, [&](uint32_t u) { return deg[u] < 2 && find(u) != rx; });
         uint64_t s;
-        const uint32_t y = g.nearest(x, [&](uint32_t u) { return find(u) != rx; }, &s);
+        if (y != UINT32_MAX) s = sq(x, y);
+        else y = g.nearest(x, [&](uint32_t u) { return find(u) != rx; }, &s);
         if (y != UINT32_MAX) pq.push(cand{s, std::min(x, y), std::max(x, y), x, y});
     };
-    for (uint32_t x = 0; x < n; ++x) query(x);
+    {
+        // every city starts as its own fragment of degree 0: its first
+        // candidate is its nearest city
+        std::vector<cand> init;
+        init.reserve(n);
+        for (uint32_t x = 0; x < n; ++x) {
+            const uint32_t y = nl.cnt[x] ? nl.ids[(size_t)x * L2O_KNN] : UINT32_MAX;
+            if (y != UINT32_MAX) init.push_back(cand{sq(x, y), std::min(x, y), std::max(x, y), x, y});
+        }
+        pq = std::priority_queue<cand, std::vector<cand>, std::greater<cand>>(std::greater<cand>(), std::move(init));
+    }
     uint32_t added = 0;
     while (added + 1 < n) {
         if (pq.empty()) return LOGO2OPT_ESTATE;  // cannot happen while two fragments remain

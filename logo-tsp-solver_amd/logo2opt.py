@@ -519,7 +519,7 @@ def _bind_server(L):
         return L
     L.logo2opt_server_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(vp)]
     L.logo2opt_server_destroy.argtypes = [vp]
-    L.logo2opt_server_submit.argtypes = [vp, C.c_uint32, vp, u32p, C.POINTER(Job), C.POINTER(C.c_int64)]
+    L.logo2opt_server_submit.argtypes = [vp, C.c_uint32, vp, vp, C.POINTER(Job), C.POINTER(C.c_int64)]
     L.logo2opt_server_pump.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.logo2opt_server_poll.argtypes = [vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
     L.logo2opt_server_result.argtypes = [vp, C.c_int64, vp, C.POINTER(Result), C.POINTER(C.c_double),
@@ -555,11 +555,16 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D):
-        route = np.ascontiguousarray(route, np.uint32)
+    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D, n=None):
+        """route None: the server builds the greedy NN tour from city 0 (give n)."""
         j = Job(ils_rounds, kick, seed, 0, metric)
         rid = C.c_int64()
-        rc = load().logo2opt_server_submit(self.h, len(route), C.cast(coords, vp), route, C.byref(j), C.byref(rid))
+        if route is None:
+            rc = load().logo2opt_server_submit(self.h, n, C.cast(coords, vp), None, C.byref(j), C.byref(rid))
+        else:
+            route = np.ascontiguousarray(route, np.uint32)
+            rc = load().logo2opt_server_submit(self.h, len(route), C.cast(coords, vp), route.ctypes.data, C.byref(j),
+                                               C.byref(rid))
         if rc == -6:
             return None
         _check(rc, "logo2opt_server_submit")
@@ -577,13 +582,18 @@ class Server:
         return [buf[k] for k in range(cnt.value)]
 
     def result(self, rid, n):
+        """A finished request's result; for a request rejected at admission
+        {"error": code, "latency_s": ...} (e.g. -4, LOGO2OPT_ENOMEM: larger
+        than the whole HBM budget)."""
         r = np.zeros(n, np.uint32)
         res = Result()
         lat, qs = C.c_double(), C.c_double()
-        _check(load().logo2opt_server_result(self.h, rid, r.ctypes.data, C.byref(res), C.byref(lat), C.byref(qs)),
-               "server_result")
+        rc = load().logo2opt_server_result(self.h, rid, r.ctypes.data, C.byref(res), C.byref(lat), C.byref(qs))
+        if rc in (-4, -2, -3):  # rejected at admission: ENOMEM, ECOORD, ETOUR
+            return {"error": rc, "latency_s": lat.value}
+        _check(rc, "server_result")
         out = {f: getattr(res, f) for f, _ in Result._fields_}
-        out.update(route=r, latency_s=lat.value, queue_s=qs.value)
+        out.update(route=r, latency_s=lat.value, queue_s=qs.value, error=0)
         return out
 
     def set_limits(self, max_running=0, max_inflight_pairs=0):
@@ -598,13 +608,15 @@ class Server:
         return dict(zip(["submitted", "rejected", "admitted", "done", "queued", "running"], c.tolist()))
 
     def replay(self, arrivals, requests, horizon_s, steps=32, sync_every=16):
-        """requests: list of (coords, route, Job).  Returns (latency[], queue[], elapsed)."""
+        """requests: list of (coords, route, Job), or (coords, None, Job, n) to
+        let the server build the NN start tour.  Returns (latency[], queue[],
+        elapsed)."""
         m = len(requests)
         arr = np.ascontiguousarray(arrivals, np.float64)
-        ns = np.array([len(r[1]) for r in requests], np.uint32)
+        ns = np.array([len(r[1]) if r[1] is not None else r[3] for r in requests], np.uint32)
         cptr = (vp * m)(*[C.cast(r[0], vp) for r in requests])
-        routes = [np.ascontiguousarray(r[1], np.uint32) for r in requests]
-        rptr = (vp * m)(*[r.ctypes.data for r in routes])
+        routes = [np.ascontiguousarray(r[1], np.uint32) if r[1] is not None else None for r in requests]
+        rptr = (vp * m)(*[r.ctypes.data if r is not None else None for r in routes])
         jobs = (Job * m)(*[r[2] for r in requests])
         lat = np.zeros(m, np.float64)
         qs = np.zeros(m, np.float64)

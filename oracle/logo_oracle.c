@@ -225,6 +225,161 @@ int orc_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route)
     return 0;
 }
 
+/* Greedy edge in two phases, the same tour as orc_mf_tour for large n:
+ *  1. every pair with s <= R^2 (found through a uniform grid of cells at
+ *     least R wide), sorted by (s, lower id, higher id), under the greedy
+ *     rule;
+ *  2. afterwards a valid edge can only join two cities of degree < 2 in
+ *     different fragments, and a pair of them not yet accepted is either
+ *     longer than R or in one fragment (it was rejected while both had
+ *     degree < 2, and fragments only merge): so all pairs among the cities
+ *     of degree < 2, sorted the same way, finish the greedy order exactly.
+ * R = 0 leaves everything to phase 2 (orc_mf_tour itself).  Returns 0, or -1
+ * when phase 2 would hold more than max_pairs pairs (call again with a larger
+ * R) or on allocation failure. */
+int orc_mf_tour_radius(uint32_t n, const int32_t *xy, uint64_t R, uint64_t max_pairs, uint32_t *route)
+{
+    if (n < 3) return -1;
+    uint32_t *uf = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *adj = (uint32_t *)malloc(sizeof(uint32_t) * 2 * n);
+    unsigned char *deg = (unsigned char *)calloc(n, 1);
+    mf_edge *e = NULL;
+    uint32_t *cell_start = NULL, *order = NULL, *rest = NULL;
+    int rc = -1;
+    if (!uf || !adj || !deg) goto out;
+    for (uint32_t v = 0; v < n; v++) uf[v] = v;
+    uint32_t added = 0;
+    if (R > 0) {
+        int64_t mnx = xy[0], mny = xy[1], mxx = xy[0], mxy = xy[1];
+        for (uint32_t v = 1; v < n; v++) {
+            if (xy[2 * v] < mnx) mnx = xy[2 * v];
+            if (xy[2 * v] > mxx) mxx = xy[2 * v];
+            if (xy[2 * v + 1] < mny) mny = xy[2 * v + 1];
+            if (xy[2 * v + 1] > mxy) mxy = xy[2 * v + 1];
+        }
+        /* cells at least R wide and at most ~4n of them */
+        int64_t cw = (int64_t)R;
+        int64_t span = (mxx - mnx > mxy - mny ? mxx - mnx : mxy - mny) + 1;
+        int64_t minw = span / (int64_t)(2.0 * sqrt((double)n)) + 1;
+        if (cw < minw) cw = minw;
+        int64_t gx = (mxx - mnx) / cw + 1, gy = (mxy - mny) / cw + 1;
+        cell_start = (uint32_t *)calloc((size_t)(gx * gy + 1), sizeof(uint32_t));
+        order = (uint32_t *)malloc(sizeof(uint32_t) * n);
+        if (!cell_start || !order) goto out;
+        for (uint32_t v = 0; v < n; v++)
+            cell_start[((xy[2 * v + 1] - mny) / cw) * gx + (xy[2 * v] - mnx) / cw + 1]++;
+        for (int64_t c = 0; c < gx * gy; c++) cell_start[c + 1] += cell_start[c];
+        {
+            uint32_t *fill = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(gx * gy));
+            if (!fill) goto out;
+            memcpy(fill, cell_start, sizeof(uint32_t) * (size_t)(gx * gy));
+            for (uint32_t v = 0; v < n; v++) order[fill[((xy[2 * v + 1] - mny) / cw) * gx + (xy[2 * v] - mnx) / cw]++] = v;
+            free(fill);
+        }
+        const uint64_t R2 = R * R;
+        uint64_t cap = 1024, m = 0;
+        e = (mf_edge *)malloc(sizeof(mf_edge) * cap);
+        if (!e) goto out;
+        for (uint32_t a = 0; a < n; a++) {
+            const int64_t cx = (xy[2 * a] - mnx) / cw, cy = (xy[2 * a + 1] - mny) / cw;
+            for (int64_t y = cy - 1; y <= cy + 1; y++) {
+                if (y < 0 || y >= gy) continue;
+                for (int64_t x = cx - 1; x <= cx + 1; x++) {
+                    if (x < 0 || x >= gx) continue;
+                    for (uint32_t q = cell_start[y * gx + x]; q < cell_start[y * gx + x + 1]; q++) {
+                        const uint32_t b = order[q];
+                        if (b <= a) continue;
+                        int64_t dx = (int64_t)xy[2 * a] - xy[2 * b], dy = (int64_t)xy[2 * a + 1] - xy[2 * b + 1];
+                        const uint64_t sq = (uint64_t)(dx * dx + dy * dy);
+                        if (sq > R2) continue;
+                        if (m == cap) {
+                            cap *= 2;
+                            mf_edge *ne = (mf_edge *)realloc(e, sizeof(mf_edge) * cap);
+                            if (!ne) goto out;
+                            e = ne;
+                        }
+                        e[m].s = sq;
+                        e[m].a = a;
+                        e[m].b = b;
+                        m++;
+                    }
+                }
+            }
+        }
+        qsort(e, m, sizeof(mf_edge), mf_cmp);
+        for (uint64_t q = 0; q < m && added + 1 < n; q++) {
+            uint32_t a = e[q].a, b = e[q].b;
+            if (deg[a] == 2 || deg[b] == 2) continue;
+            uint32_t ra = uf_find(uf, a), rb = uf_find(uf, b);
+            if (ra == rb) continue;
+            adj[2 * a + deg[a]++] = b;
+            adj[2 * b + deg[b]++] = a;
+            uf[ra] = rb;
+            added++;
+        }
+        free(e);
+        e = NULL;
+    }
+    if (added + 1 < n) {
+        uint32_t nr = 0;
+        rest = (uint32_t *)malloc(sizeof(uint32_t) * n);
+        if (!rest) goto out;
+        for (uint32_t v = 0; v < n; v++)
+            if (deg[v] < 2) rest[nr++] = v;
+        const uint64_t m = (uint64_t)nr * (nr - 1) / 2;
+        if (m > max_pairs) goto out;
+        e = (mf_edge *)malloc(sizeof(mf_edge) * (m ? m : 1));
+        if (!e) goto out;
+        uint64_t k = 0;
+        for (uint32_t i = 0; i < nr; i++)
+            for (uint32_t j = i + 1; j < nr; j++) {
+                const uint32_t a = rest[i], b = rest[j];
+                int64_t dx = (int64_t)xy[2 * a] - xy[2 * b], dy = (int64_t)xy[2 * a + 1] - xy[2 * b + 1];
+                e[k].s = (uint64_t)(dx * dx + dy * dy);
+                e[k].a = a;
+                e[k].b = b;
+                k++;
+            }
+        qsort(e, m, sizeof(mf_edge), mf_cmp);
+        for (uint64_t q = 0; q < m && added + 1 < n; q++) {
+            uint32_t a = e[q].a, b = e[q].b;
+            if (deg[a] == 2 || deg[b] == 2) continue;
+            uint32_t ra = uf_find(uf, a), rb = uf_find(uf, b);
+            if (ra == rb) continue;
+            adj[2 * a + deg[a]++] = b;
+            adj[2 * b + deg[b]++] = a;
+            uf[ra] = rb;
+            added++;
+        }
+    }
+    {
+        uint32_t ends[2], ne = 0;
+        for (uint32_t v = 0; v < n && ne < 2; v++)
+            if (deg[v] < 2) ends[ne++] = v;
+        if (ne != 2) goto out;
+        adj[2 * ends[0] + deg[ends[0]]++] = ends[1];
+        adj[2 * ends[1] + deg[ends[1]]++] = ends[0];
+        uint32_t prev = 0, cur = adj[0] < adj[1] ? adj[0] : adj[1];
+        route[0] = 0;
+        for (uint32_t i = 1; i < n; i++) {
+            route[i] = cur;
+            uint32_t nx = adj[2 * cur] == prev ? adj[2 * cur + 1] : adj[2 * cur];
+            prev = cur;
+            cur = nx;
+        }
+    }
+    rc = 0;
+out:
+    free(e);
+    free(cell_start);
+    free(order);
+    free(rest);
+    free(uf);
+    free(adj);
+    free(deg);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Distances                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -462,11 +617,19 @@ void orc_apply(uint32_t n, uint32_t *route, uint32_t i, uint32_t j)
     free(tmp);
 }
 
+/* The sweep of the loops below: the plain scan (orc_step_m), or the exact
+ * SIMD sweep (orc_step_fast), which returns the same move and makes full-size
+ * runs (tens of thousands of cities, thousands of sweeps) take minutes. */
+static int loop_step(int fast, uint32_t n, const int32_t *xy, const uint32_t *route, int threads, int metric,
+                     orc_best2_out *o)
+{
+    return fast ? orc_step_fast(n, xy, route, threads, metric, o, NULL) : orc_step_m(n, xy, route, threads, metric, o);
+}
+
 /* twoOptLocalSolver.cpp:22-64: step, apply while the gain is negative. */
-uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
-                            int threads, int metric, uint64_t max_moves,
-                            orc_best2_out *log, uint64_t log_cap,
-                            uint64_t *sweeps, uint64_t *length)
+static uint64_t ls_core(uint32_t n, const int32_t *xy, uint32_t *route, int threads, int metric, int fast,
+                        uint64_t max_moves, orc_best2_out *log, uint64_t log_cap, uint64_t *sweeps,
+                        uint64_t *length)
 {
     uint64_t moves = 0, sw = 0;
     uint64_t L = orc_route_length_m(n, xy, route, metric);
@@ -474,7 +637,7 @@ uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
         if (max_moves && moves >= max_moves) break;
         orc_best2_out o;
         sw++;
-        if (!orc_step_m(n, xy, route, threads, metric, &o)) break;
+        if (!loop_step(fast, n, xy, route, threads, metric, &o)) break;
         orc_apply(n, route, o.i, o.j);
         L = (uint64_t)((int64_t)L + o.minchange);
         if (log && moves < log_cap) log[moves] = o;
@@ -483,6 +646,22 @@ uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
     if (sweeps) *sweeps = sw;
     if (length) *length = L;
     return moves;
+}
+
+uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
+                            int threads, int metric, uint64_t max_moves,
+                            orc_best2_out *log, uint64_t log_cap,
+                            uint64_t *sweeps, uint64_t *length)
+{
+    return ls_core(n, xy, route, threads, metric, 0, max_moves, log, log_cap, sweeps, length);
+}
+
+uint64_t orc_local_search_fast(uint32_t n, const int32_t *xy, uint32_t *route,
+                               int threads, int metric, uint64_t max_moves,
+                               orc_best2_out *log, uint64_t log_cap,
+                               uint64_t *sweeps, uint64_t *length)
+{
+    return ls_core(n, xy, route, threads, metric, 1, max_moves, log, log_cap, sweeps, length);
 }
 
 uint64_t orc_local_search(uint32_t n, const int32_t *xy, uint32_t *route,
@@ -551,14 +730,14 @@ void orc_ref_kick(uint32_t n, uint32_t *route, orc_rng_t *rng)
 /* ILSGlobalSolverSequential.cpp:60-100 with a fixed round count in place of
  * the target/time stop rule: LO, accept iff strictly shorter else revert to
  * the best, kick the best. */
-uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
-                   uint32_t rounds, int kick_kind, int threads, int metric,
-                   uint64_t *round_lengths, uint64_t *total_moves)
+static uint64_t ils_core(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed, uint32_t rounds,
+                         int kick_kind, int threads, int metric, int fast, uint64_t *round_lengths,
+                         uint64_t *total_moves)
 {
     orc_rng_t rng;
     orc_rng_seed(&rng, seed);
     uint64_t L, tm = 0;
-    tm += orc_local_search_m(n, xy, route, threads, metric, 0, NULL, 0, NULL, &L);
+    tm += ls_core(n, xy, route, threads, metric, fast, 0, NULL, 0, NULL, &L);
     uint64_t Lbest = L;
     uint32_t *best = (uint32_t *)malloc(sizeof(uint32_t) * n);
     uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * n);
@@ -570,7 +749,7 @@ uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed
             orc_ref_kick(n, cur, &rng);
         else
             orc_double_bridge(n, cur, &rng, NULL);
-        tm += orc_local_search_m(n, xy, cur, threads, metric, 0, NULL, 0, NULL, &L);
+        tm += ls_core(n, xy, cur, threads, metric, fast, 0, NULL, 0, NULL, &L);
         if (L < Lbest) {
             Lbest = L;
             memcpy(best, cur, sizeof(uint32_t) * n);
@@ -582,6 +761,20 @@ uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed
     free(cur);
     if (total_moves) *total_moves = tm;
     return Lbest;
+}
+
+uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                   uint32_t rounds, int kick_kind, int threads, int metric,
+                   uint64_t *round_lengths, uint64_t *total_moves)
+{
+    return ils_core(n, xy, route, seed, rounds, kick_kind, threads, metric, 0, round_lengths, total_moves);
+}
+
+uint64_t orc_ils_fast(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                      uint32_t rounds, int kick_kind, int threads, int metric,
+                      uint64_t *round_lengths, uint64_t *total_moves)
+{
+    return ils_core(n, xy, route, seed, rounds, kick_kind, threads, metric, 1, round_lengths, total_moves);
 }
 
 uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,

@@ -63,6 +63,9 @@ void orc_random_perm(uint32_t n, uint64_t seed, uint32_t *route);
 void orc_nn_tour(uint32_t n, const int32_t *xy, uint32_t start, uint32_t *route);
 /* greedy edge (multiple fragment) by sorting all edges; O(n^2 log n) */
 int  orc_mf_tour(uint32_t n, const int32_t *xy, uint32_t *route);
+/* the same tour for large n: pairs within R first, then all pairs among the
+ * cities left with degree < 2 (-1 if those exceed max_pairs) */
+int  orc_mf_tour_radius(uint32_t n, const int32_t *xy, uint64_t R, uint64_t max_pairs, uint32_t *route);
 
 /* ---- distances -------------------------------------------------------- */
 /* nint(sqrt(s)) = k + [s > k^2 + k] with k = isqrt(s) */
@@ -112,6 +115,13 @@ uint64_t orc_local_search_m(uint32_t n, const int32_t *xy, uint32_t *route,
                             orc_best2_out *log, uint64_t log_cap,
                             uint64_t *sweeps, uint64_t *length);
 
+/* orc_local_search_m with the exact SIMD sweep (orc_step_fast) in place of
+ * the plain scan: the same moves, fast enough for full-size runs. */
+uint64_t orc_local_search_fast(uint32_t n, const int32_t *xy, uint32_t *route,
+                               int threads, int metric, uint64_t max_moves,
+                               orc_best2_out *log, uint64_t log_cap,
+                               uint64_t *sweeps, uint64_t *length);
+
 /* Double-bridge A B C D -> A C B D with 1 <= p1 < p2 < p3 <= n-1 drawn from
  * the generator.  cuts (optional) receives p1,p2,p3. */
 void orc_double_bridge(uint32_t n, uint32_t *route, orc_rng_t *rng, uint32_t *cuts);
@@ -135,6 +145,11 @@ uint64_t orc_ils(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
 uint64_t orc_ils_m(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
                    uint32_t rounds, int kick_kind, int threads, int metric,
                    uint64_t *round_lengths, uint64_t *total_moves);
+
+/* orc_ils_m with the exact SIMD sweep (orc_step_fast). */
+uint64_t orc_ils_fast(uint32_t n, const int32_t *xy, uint32_t *route, uint64_t seed,
+                      uint32_t rounds, int kick_kind, int threads, int metric,
+                      uint64_t *round_lengths, uint64_t *total_moves);
 
 /* Multi-start ILS with a shared best, rounds synchronised across chains
  * (ILSGlobalSolverMT.cpp:86-195 with comm = 1); chain c seeds seed + c. */

@@ -74,6 +74,11 @@ def _load_oracle():
     L.orc_step_fast.argtypes = [C.c_uint32, i32p, u32p, C.c_int, C.c_int, C.POINTER(Best2Out),
                                 C.POINTER(C.c_uint64)]
     L.orc_fast_isa.restype = C.c_char_p
+    L.orc_mf_tour_radius.argtypes = [C.c_uint32, i32p, C.c_uint64, C.c_uint64, u32p]
+    L.orc_local_search_fast.argtypes = L.orc_local_search_m.argtypes
+    L.orc_local_search_fast.restype = C.c_uint64
+    L.orc_ils_fast.argtypes = L.orc_ils_m.argtypes
+    L.orc_ils_fast.restype = C.c_uint64
     return L
 
 
@@ -113,6 +118,22 @@ def mf_tour(xy):
     n = len(xy) // 2
     r = np.zeros(n, np.uint32)
     assert lib.orc_mf_tour(n, xy, r) == 0
+    return r
+
+
+def mf_tour_radius(xy, radius=None, max_pairs=1 << 27):
+    """The greedy-edge tour for large n: pairs within `radius` first (default:
+    4x the mean spacing over the bounding box), then all pairs among the
+    cities left with degree < 2; the radius doubles until the second phase
+    fits max_pairs."""
+    n = len(xy) // 2
+    r = np.zeros(n, np.uint32)
+    if radius is None:
+        x, y = xy[0::2].astype(np.int64), xy[1::2].astype(np.int64)
+        area = max(1, int(x.max() - x.min() + 1)) * max(1, int(y.max() - y.min() + 1))
+        radius = max(1, int(4 * (area / n) ** 0.5))
+    while lib.orc_mf_tour_radius(n, xy, radius, max_pairs, r) != 0:
+        radius *= 2
     return r
 
 
@@ -162,23 +183,27 @@ def apply(route, i, j):
     return r
 
 
-def local_search(xy, route, threads=1, max_moves=0, log_cap=1 << 20, metric=METRIC_EUC_2D):
+def local_search(xy, route, threads=1, max_moves=0, log_cap=1 << 20, metric=METRIC_EUC_2D, fast=False):
+    """fast=True runs the loop on the exact SIMD sweep (same moves)."""
     r = np.array(route, np.uint32)
     log = np.zeros(3 * log_cap, np.int32)
     sweeps = C.c_uint64()
     length = C.c_uint64()
-    moves = lib.orc_local_search_m(len(r), xy, r, threads, metric, max_moves, log.ctypes.data, log_cap,
-                                   C.byref(sweeps), C.byref(length))
+    fn = lib.orc_local_search_fast if fast else lib.orc_local_search_m
+    moves = fn(len(r), xy, r, threads, metric, max_moves, log.ctypes.data, log_cap, C.byref(sweeps),
+               C.byref(length))
     m = min(moves, log_cap)
     return r, int(length.value), int(moves), int(sweeps.value), log[:3 * m].reshape(m, 3).copy()
 
 
-def ils(xy, route, seed, rounds, kick="double_bridge", threads=1, metric=METRIC_EUC_2D):
+def ils(xy, route, seed, rounds, kick="double_bridge", threads=1, metric=METRIC_EUC_2D, fast=False):
+    """fast=True runs the local searches on the exact SIMD sweep (same moves)."""
     r = np.array(route, np.uint32)
     rl = np.zeros(rounds + 1, np.uint64)
     tm = C.c_uint64()
-    best = lib.orc_ils_m(len(r), xy, r, seed, rounds, 0 if kick == "double_bridge" else 1, threads, metric,
-                         rl.ctypes.data, C.byref(tm))
+    fn = lib.orc_ils_fast if fast else lib.orc_ils_m
+    best = fn(len(r), xy, r, seed, rounds, 0 if kick == "double_bridge" else 1, threads, metric, rl.ctypes.data,
+              C.byref(tm))
     return r, int(best), rl, int(tm.value)
 
 

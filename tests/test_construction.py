@@ -1,6 +1,10 @@
 """Tour construction (CPU): the library's multiple-fragment (greedy edge) tour
-against the oracle's brute-force restatement (all edges sorted), and the
-greedy nearest-neighbour tour against the oracle's O(n^2) scan.
+against the oracle's brute-force restatement (all edges sorted) and, at the
+serving and config sizes, against its two-phase restatement (pairs within a
+radius, then all pairs among the cities left with degree < 2), itself pinned
+to the brute force for every radius; the greedy nearest-neighbour tour
+against the oracle's O(n^2) scan.  The library takes both from nearest-city
+lists built on several host threads, so these are the threaded host tests.
 
 Reference: routeInit's "mf" / "nn" options (construction.c:118-150, default
 "mf" in defs.h:70; multipleFragment_KD and initRoute_KD_NN,
@@ -61,3 +65,20 @@ def test_nn_matches_oracle():
 def test_mf_rejects_bad_arguments():
     with pytest.raises(L.Logo2OptError):
         L.mf_tour(np.zeros(4, np.int32))  # n = 2
+
+
+def test_mf_radius_oracle_equals_brute_force_for_every_radius():
+    for n, side, kind in [(9, 3, "u"), (64, 8, "u"), (300, 10**4, "u"), (1000, 10**6, "c"), (1200, 40, "u")]:
+        xy = O.gen_uniform(n, side, 100 + n) if kind == "u" else O.gen_clustered(n, side, 100 + n)
+        want = O.mf_tour(xy).tolist()
+        for radius in [0, 1, 2, 5, 50, 10**3, 10**5, 10**7]:
+            r = np.zeros(n, np.uint32)
+            assert O.lib.orc_mf_tour_radius(n, xy, radius, 1 << 40, r) == 0
+            assert r.tolist() == want, (n, radius)
+
+
+@pytest.mark.parametrize("n,kind,seed", [(33810, "c", 2), (50000, "c", 77), (50000, "u", 5)])
+def test_constructions_at_full_size(n, kind, seed):
+    xy = O.gen_clustered(n, 10**6, seed) if kind == "c" else O.gen_uniform(n, 10**6, seed)
+    assert L.mf_tour(xy).tolist() == O.mf_tour_radius(xy).tolist()
+    assert L.nn_tour(xy, 0).tolist() == O.nn_tour(xy, 0).tolist()

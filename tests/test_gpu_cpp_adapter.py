@@ -34,16 +34,19 @@ int main(int argc, char **argv)
     const logo2opt_best2_out first = s.optimizeStep(route);
     std::printf("step %u %u %d\n", first.i, first.j, first.minchange);
     std::vector<uint32_t> start = route;
-    std::vector<unsigned long> best;
+    // the caller's best-length list is only read, never written
+    // (twoOptLocalSolver.cpp:22-64)
+    std::vector<unsigned long> best(1, 12345ul);
     s.optimize(route, best);
-    std::printf("lo %llu %llu\n", (unsigned long long)s.routeLength(route), (unsigned long long)best.back());
+    std::printf("lo %llu %llu\n", (unsigned long long)s.routeLength(route), (unsigned long long)s.lastLength());
+    std::printf("untouched %d\n", (int)(best.size() == 1 && best[0] == 12345ul));
     {
         logo2opt::HIPSolver p(c.data(), n, 0, 64, metric);
         p.setPruning(true);
         std::vector<uint32_t> rp = start;
         std::vector<unsigned long> bp;
         p.optimize(rp, bp);
-        std::printf("pruned %llu %d\n", (unsigned long long)bp.back(), (int)(rp == route));
+        std::printf("pruned %llu %d\n", (unsigned long long)p.lastLength(), (int)(rp == route && bp.empty()));
     }
     const uint64_t ils = s.ils(route, 7, 10);
     std::printf("ils %llu %llu\n", (unsigned long long)ils, (unsigned long long)s.routeLength(route));
@@ -89,6 +92,7 @@ def test_cpp_adapter_on_tsplib_file(tmp_path, name):
     assert lines["step"] == [i, j, g]
     r, Lo, *_ = O.local_search(xy, r0, threads=4, metric=m)
     assert lines["lo"] == [Lo, Lo]
+    assert lines["untouched"] == [1]
     assert lines["pruned"] == [Lo, 1]  # same length and tour with row pruning
     rb, Lb, _, _ = O.ils(xy, r, 7, 10, threads=4, metric=m)
     assert lines["ils"] == [Lb, Lb]

@@ -88,6 +88,28 @@ def test_local_search_config1_move_sequence(sync_every):
         assert st["sweeps"] == sweeps and st["moves"] == moves
 
 
+def test_optimize_move_log_any_cap():
+    """logo2opt_optimize fills min(moves, log_cap) entries for any log_cap:
+    a short cap gives the first moves, a cap beyond the context's initial
+    buffers (2,000,000 entries) makes it grow them, the full log equals the
+    oracle's, and log_cap >= 2^32 is rejected."""
+    n = 1000
+    xy = O.gen_uniform(n, 10**4, 1)
+    r0 = O.random_perm(n, 2)
+    r, Lo, moves, sweeps, log = O.local_search(xy, r0, threads=THREADS)
+    with ctx_for(xy) as ctx:
+        short = ctx.optimize(r0, log_cap=10, sync_every=64)
+        np.testing.assert_array_equal(short["log"], log[:10])
+        full = ctx.optimize(r0, log_cap=2_000_000, sync_every=64)
+        assert full["moves"] == moves and full["length"] == Lo
+        np.testing.assert_array_equal(full["log"], log)
+        again = ctx.optimize(r0, log_cap=1 << 16, sync_every=64)  # after the growth
+        np.testing.assert_array_equal(again["log"], log)
+        rc = L.load().logo2opt_optimize(ctx.h, np.array(r0, np.uint32), None, None,
+                                        L.C.cast((L.Best2Out * 1)(), L.vp), 1 << 32, 64)
+        assert rc == -1  # LOGO2OPT_EINVAL
+
+
 @pytest.mark.parametrize("n,side,kind,start", [
     (2000, 10**6, "c", "nn"), (1500, 10**6, "u", "rand"), (3000, 2**22 - 1, "u", "nn"), (800, 40, "u", "rand"),
 ])

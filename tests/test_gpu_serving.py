@@ -103,6 +103,71 @@ def test_server_queue_cap_rejects():
         assert srv.counts()["rejected"] == 3
 
 
+def test_server_rejects_request_larger_than_budget_and_serves_the_rest():
+    """A request that needs more than the whole HBM budget is rejected at
+    admission with LOGO2OPT_ENOMEM, reported by poll, and does not block the
+    requests queued behind it (FIFO admission); the replay loop ends."""
+    small = make_requests(3, 200, 400, 23, rounds=2)
+    big = make_requests(1, 6000, 6001, 29, rounds=2)[0]
+    budget = 3 * max(L.Engine.bytes_for(rq["n"], ils_rounds=2) for rq in small) + 4096
+    assert L.Engine.bytes_for(big["n"], ils_rounds=2) > budget
+    with L.Server(hbm_budget=budget, max_slots=8) as srv:
+        rid_big = srv.submit(big["coords"], big["route"], ils_rounds=2, seed=big["seed"])
+        ids = [srv.submit(rq["coords"], rq["route"], ils_rounds=2, seed=rq["seed"]) for rq in small]
+        done = []
+        for _ in range(10000):
+            srv.pump(steps=64, sync_every=16)
+            done += srv.poll()
+            if len(done) == 4:
+                break
+        assert sorted(done) == sorted([rid_big] + ids)
+        assert srv.result(rid_big, big["n"])["error"] == -4  # LOGO2OPT_ENOMEM
+        assert srv.counts()["rejected"] == 1
+        for rid, rq in zip(ids, small):
+            r, best = alone(rq)
+            assert srv.result(rid, rq["n"])["length"] == best
+    # the open-loop replay finishes with the rejected request marked -1
+    with L.Server(hbm_budget=budget, max_slots=8) as srv:
+        reqs = [big] + small
+        jobs = [L.Job(2, 0, rq["seed"], 0, 0) for rq in reqs]
+        lat, qs, el = srv.replay(np.arange(len(reqs)) * 0.001, [(rq["coords"], rq["route"], j)
+                                                               for rq, j in zip(reqs, jobs)],
+                                 horizon_s=120.0, steps=32, sync_every=16)
+    assert lat[0] == -1.0 and (lat[1:] >= 0).all() and el < 60.0
+
+
+def test_server_mix_up_to_50000_clustered_cities():
+    """Serving at the top of the serving workload's size range: a 50,000-city
+    clustered ILS request whose NN start the server builds itself, admitted
+    with small requests arriving around it; every result is bit-identical to
+    the same request run alone, and the small ones equal the oracle's ILS."""
+    big_xy = L.gen_clustered(50000, 10**6, 77)
+    big = dict(n=50000, xy=big_xy, coords=L.coords_of(big_xy), route=None, seed=4242, rounds=3)
+    small = make_requests(4, 300, 2000, 31, rounds=3)
+    with L.Server(hbm_budget=1 << 30, max_slots=16) as srv:
+        rid_big = srv.submit(big["coords"], None, ils_rounds=3, seed=big["seed"], n=big["n"])
+        ids = [srv.submit(small[0]["coords"], small[0]["route"], ils_rounds=3, seed=small[0]["seed"])]
+        srv.pump(steps=8, sync_every=8)
+        ids += [srv.submit(rq["coords"], rq["route"], ils_rounds=3, seed=rq["seed"]) for rq in small[1:]]
+        done = []
+        while len(done) < 5:
+            srv.pump(steps=64, sync_every=16)
+            done += srv.poll()
+        got_big = srv.result(rid_big, big["n"])
+        got = [srv.result(rid, rq["n"]) for rid, rq in zip(ids, small)]
+    r0 = L.nn_tour(big_xy, 0)
+    with L.Context(big["coords"]) as ctx:
+        r, best, rl = ctx.ils(r0, seed=big["seed"], rounds=3)
+    assert got_big["length"] == best and got_big["route"].tolist() == r.tolist()
+    say(f"50,000 clustered cities served: {best}")
+    for g, rq in zip(got, small):
+        r, best = alone(rq)
+        assert g["length"] == best and g["route"].tolist() == r.tolist()
+        rb, Lb, _, _ = O.ils(np.ascontiguousarray(rq["xy"], np.int32), rq["route"], rq["seed"], rq["rounds"],
+                             threads=THREADS)
+        assert Lb == best
+
+
 def test_server_open_loop_replay():
     reqs = make_requests(16, 200, 2500, 21, rounds=3)
     rs = np.random.default_rng(3)
