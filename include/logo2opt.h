@@ -61,6 +61,7 @@ typedef struct {                                                      /* defs.h:
 #define LOGO2OPT_ENOTFOUND  -8  /* unknown slot / request id                   */
 #define LOGO2OPT_EFORMAT    -9  /* not a supported TSPLIB .tsp / .tour file    */
 #define LOGO2OPT_EIO       -10  /* file could not be opened, read or written   */
+#define LOGO2OPT_ESIZE     -11  /* n above LOGO2OPT_NOALLOC_MAX_N (no-alloc call) */
 const char *logo2opt_strerror(int code);
 
 #define LOGO2OPT_MAX_N (1u << 20)
@@ -139,6 +140,19 @@ int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32
  * rounds >= 1, 1 <= chains <= 4096. */
 int logo2opt_ils_multistart(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds,
                             uint32_t chains, int kick, uint64_t *best_length, uint64_t *round_lengths);
+
+/* logo2opt_step without a context and without allocating memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same inputs, checks and result as logo2opt_create_metric +
+ * logo2opt_step; the device work uses one fixed array of the library (about
+ * 3 MiB per device, held until the process ends) instead of a context's
+ * buffers, so calls that overlap on one device race: serialise them.
+ * n <= LOGO2OPT_NOALLOC_MAX_N, else LOGO2OPT_ESIZE.  Synchronous (returns
+ * when *out is written). */
+#define LOGO2OPT_NOALLOC_MAX_N 65536u
+int logo2opt_step_noalloc(int device, uint32_t n, const logo2opt_city_coords *coords, int metric,
+                          const uint32_t *route, logo2opt_best2_out *out);
 
 /* routeLength (src/headers.h:170-189), exact, computed on the device. */
 int logo2opt_route_length(logo2opt_ctx *ctx, const uint32_t *route, uint64_t *length);

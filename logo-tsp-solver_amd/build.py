@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "liblogo2opt.so")
 SOURCES = ["csrc/l2o_kernels.hip", "csrc/l2o_engine.hip", "csrc/l2o_workload.cpp", "csrc/l2o_server.cpp",
-           "csrc/l2o_tsplib.cpp"]
+           "csrc/l2o_tsplib.cpp", "csrc/l2o_noalloc.hip"]
 ARCH = os.environ.get("LOGO2OPT_ARCH", "gfx950")
 
 

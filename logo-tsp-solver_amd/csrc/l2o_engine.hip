@@ -1006,6 +1006,7 @@ const char *logo2opt_strerror(int code)
     case LOGO2OPT_ENOTFOUND: return "unknown slot";
     case LOGO2OPT_EFORMAT: return "not a supported TSPLIB file";
     case LOGO2OPT_EIO: return "file could not be opened, read or written";
+    case LOGO2OPT_ESIZE: return "more cities than the no-allocation entry point's fixed array holds";
     default: return "unknown error";
     }
 }

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "liblogo2opt.so")
 KICK_DOUBLE_BRIDGE = 0
 KICK_REFERENCE = 1
 MAX_N = 1 << 20
+NOALLOC_MAX_N = 1 << 16  # LOGO2OPT_NOALLOC_MAX_N
 METRIC_EUC_2D = 0
 METRIC_CEIL_2D = 1
 METRIC_ATT = 2
@@ -40,6 +41,7 @@ EXPORTS = [
     "logo2opt_tuning_shapes", "logo2opt_tuning_set", "logo2opt_tuning_get", "logo2opt_tuning_load",
     "logo2opt_engine_time_sweep", "logo2opt_create_metric", "logo2opt_tsp_read", "logo2opt_tour_read",
     "logo2opt_tour_write", "logo2opt_host_route_length", "logo2opt_mf_tour", "logo2opt_ils_multistart",
+    "logo2opt_step_noalloc",
     "logo2opt_engine_add_group", "logo2opt_engine_group_result", "logo2opt_set_pruning",
     "logo2opt_engine_set_pruning",
 ]
