@@ -54,6 +54,13 @@ FLOP_PER_EVAL = 14
 PROFILED_TRAFFIC = {"l2o::k_sweep<6,160,1,false>": 3339.72 * 1024 * 2.0 + 5.23 * 1024 * 1.0}
 REF_FLOP_PER_EVAL = 36    # the reference's GPU GFLOPS convention (solver.c:305-306)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+# The sweep's own bound: VALU instruction issue.  Each of the 1,024 SIMDs
+# (256 CUs x 4) issues one wave64 VALU instruction per 4 cycles at 2.4 GHz
+# (the FP32 peak above is this rate x 64 lanes x 2 for FMA x 2 for packed
+# FP32).  VALU instructions per candidate come from SQ_INSTS_VALU of the
+# profiled kernel (per dispatch / candidates per dispatch, wave instructions).
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
+PROFILED_VALU_PER_EVAL = {"l2o::k_sweep<6,160,1,false>": 3.633e7 / 571473528}
 
 
 def log(*a):
@@ -449,6 +456,14 @@ def main():
                 "kernel": kname, "avg_kernel_ms": avg_ms, "launches": launches,
                 "evals_per_s_kernel": pairs(n) * work_sweeps / launches / (avg_ms * 1e-3),
                 "ref_convention_tflops": achieved * REF_FLOP_PER_EVAL / FLOP_PER_EVAL}
+    vpe = PROFILED_VALU_PER_EVAL.get(kname)
+    if vpe:
+        rate = roofline["evals_per_s_kernel"] * vpe
+        roofline["valu_issue"] = {
+            "achieved": rate, "peak": VALU_ISSUE_PEAK, "unit": "wave64 VALU instructions/s",
+            "frac": rate / VALU_ISSUE_PEAK, "valu_per_eval_x64": vpe * 64,
+            "note": "the kernel's real bound: integer VALU issue (SQ_INSTS_VALU per candidate from the profiled "
+                    "counters x candidates/s in the kernel); frac above is the 14-FLOP contract figure"}
     ctx.close()
 
     pruned = None

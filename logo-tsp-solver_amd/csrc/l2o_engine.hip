@@ -32,10 +32,17 @@
 
 using l2o::sweep_cfg;
 
-#define L2O_TRY(call)                                     \
-    do {                                                  \
-        hipError_t e_ = (call);                           \
-        if (e_ != hipSuccess) return LOGO2OPT_EDEVICE;    \
+// a failed HIP call returns LOGO2OPT_EDEVICE; with LOGO2OPT_DEBUG set in the
+// environment it is also reported on stderr (call, file:line, HIP's message)
+#define L2O_TRY(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            if (std::getenv("LOGO2OPT_DEBUG"))                                                     \
+                std::fprintf(stderr, "logo2opt: %s (%s:%d): %s\n", #call, __FILE__, __LINE__,      \
+                             hipGetErrorString(e_));                                               \
+            return LOGO2OPT_EDEVICE;                                                               \
+        }                                                                                          \
     } while (0)
 
 namespace {
@@ -223,19 +230,12 @@ struct logo2opt_engine {
     struct group {
         sweep_cfg cfg;
         uint32_t off, count, id;
-        bool prune;            // launched as the row-pruning variant
-        uint32_t roff = 0;     // ranges mode: first entry in d_ranges
-        uint32_t nblocks = 0;  // ranges mode: workgroups (0: one workgroup per item)
+        bool prune;  // launched as the row-pruning variant
     };
     std::vector<group> groups;
     l2o_sweep_item *d_sitems = nullptr;
     size_t sitems_cap = 0;
-    uint32_t *d_ranges = nullptr;
-    size_t ranges_cap = 0;
-    // sweep launch shape: L2O_RANGES_OFF one workgroup per work item (the
-    // dispatcher balances), _FULL a grid of resident workgroups with
-    // host-balanced item ranges for full sweeps, _ALL also for pruned sweeps
-    int ranges_mode = 0;
+
     l2o_apply_item *d_aitems = nullptr;
     size_t aitems_cap = 0;
     uint32_t n_aitems = 0;
@@ -444,93 +444,6 @@ uint64_t item_work(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
     return w;
 }
 
-enum { L2O_RANGES_OFF = 0, L2O_RANGES_FULL = 1, L2O_RANGES_ALL = 2 };
-
-// Per-item fixed cost in wave-rows (the prologue: item and instance loads,
-// column loads, the offset scan, the shared key), for balancing ranges.
-#ifndef L2O_ITEM_COST_ROWS
-#define L2O_ITEM_COST_ROWS 12
-#endif
-// a group gets ranges only when every resident workgroup would have at
-// least this many wave-rows to evaluate
-#ifndef L2O_RANGES_MIN_ROWS
-#define L2O_RANGES_MIN_ROWS 64
-#endif
-
-// VALU cost of an item: rows stepped by each of its waves (the kernel's own
-// row limit, including the cut at the diagonal), plus the prologue per wave
-double item_cost(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
-{
-    const int WC = 64 * c.k - 1, a0 = (int)it.a0;
-    double s = 0.0;
-    for (int w = 0; w < c.waves; ++w) {
-        const int bw = (int)it.b0 + w * WC;
-        if (bw > (int)n - 2) break;
-        int t_end = std::min((int)it.rows, (int)n - 3 - a0);
-        if (bw < a0 + t_end + 1) t_end = std::min(t_end, std::min(bw + WC - 1, (int)n - 2) - 1 - a0);
-        if (t_end > 0) s += t_end + L2O_ITEM_COST_ROWS;
-    }
-    return s;
-}
-
-// Splits the items v (in order) into G contiguous ranges of equal cost;
-// an item a range boundary falls in is cut at a row that is a multiple of 4
-// from its start (both pieces are items of their own).  out receives the
-// items, ranges the G+1 range starts.
-void split_ranges(const logo2opt_engine *e, const std::vector<l2o_sweep_item> &v, const sweep_cfg &c, uint32_t G,
-                  std::vector<l2o_sweep_item> &out, std::vector<uint32_t> &ranges)
-{
-    auto cost = [&](const l2o_sweep_item &it) { return item_cost(it, c, e->slots[it.inst].n); };
-    double total = 0.0;
-    for (const auto &it : v) total += cost(it);
-    const double Q = total / G;
-    out.clear();
-    ranges.assign(1, 0u);
-    double cum = 0.0;
-    uint32_t g = 0;
-    for (l2o_sweep_item it : v) {
-        for (;;) {
-            const double w = cost(it), bound = (g + 1) * Q;
-            if (g + 1 >= G || cum + w <= bound) {
-                out.push_back(it);
-                cum += w;
-                break;
-            }
-            // the boundary falls inside this item: close the range before it,
-            // after it, or after a cut at 4r rows, whichever ends nearest
-            uint32_t best_r = 0;
-            double best_d = bound - cum;
-            for (uint32_t r = 4; r < it.rows; r += 4) {
-                l2o_sweep_item a = it;
-                a.rows = r;
-                const double d = std::fabs(cum + cost(a) - bound);
-                if (d < best_d) {
-                    best_d = d;
-                    best_r = r;
-                }
-            }
-            if (std::fabs(cum + w - bound) <= best_d) {
-                out.push_back(it);
-                cum += w;
-                ranges.push_back((uint32_t)out.size());
-                ++g;
-                break;
-            }
-            if (best_r) {
-                l2o_sweep_item a = it;
-                a.rows = best_r;
-                out.push_back(a);
-                cum += cost(a);
-                it.a0 += best_r;
-                it.rows -= best_r;
-            }
-            ranges.push_back((uint32_t)out.size());
-            ++g;
-        }
-    }
-    while (ranges.size() < (size_t)G + 1) ranges.push_back((uint32_t)out.size());
-}
-
 int rebuild_lists(logo2opt_engine *e)
 {
     std::map<std::tuple<int, int, int, bool>, std::vector<l2o_sweep_item>> by_cfg;
@@ -565,33 +478,10 @@ int rebuild_lists(logo2opt_engine *e)
         if (s.group_lead != L2O_NO_GROUP) e->group_grid = std::max(e->group_grid, id + 1);
     }
     std::vector<l2o_sweep_item> all;
-    std::vector<uint32_t> all_ranges;
     e->groups.clear();
     for (auto &kv : by_cfg) {
         const sweep_cfg c{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
         const bool pr = std::get<3>(kv.first);
-        if (e->ranges_mode == L2O_RANGES_ALL || (e->ranges_mode == L2O_RANGES_FULL && !pr)) {
-            // a grid of resident workgroups, each with a contiguous range of
-            // equal cost (items in row-block order, so a range's items share
-            // rows and the diagonal items spread over the ranges)
-            const int G = l2o::sweep_ranges_resident(c, pr);
-            double total = 0.0;
-            for (const auto &it : kv.second) total += item_cost(it, c, e->slots[it.inst].n);
-            if (G > 0 && total >= (double)G * L2O_RANGES_MIN_ROWS) {
-                std::vector<l2o_sweep_item> items;
-                std::vector<uint32_t> ranges;
-                split_ranges(e, kv.second, c, (uint32_t)G, items, ranges);
-                if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;
-                logo2opt_engine::group gr{c, (uint32_t)all.size(), (uint32_t)items.size(), (uint32_t)e->groups.size(),
-                                          pr};
-                gr.roff = (uint32_t)all_ranges.size();
-                gr.nblocks = (uint32_t)G;
-                e->groups.push_back(gr);
-                all.insert(all.end(), items.begin(), items.end());
-                all_ranges.insert(all_ranges.end(), ranges.begin(), ranges.end());
-                continue;
-            }
-        }
         // Items are dispatched in list order (largest instances first), so at
         // any moment the device works on a window of consecutive items of one
         // or a few instances, whose points stay hot in every XCD's L2.
@@ -632,14 +522,6 @@ int rebuild_lists(logo2opt_engine *e)
         e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size(), pr});
         all.insert(all.end(), v.begin(), v.end());
     }
-    if (all_ranges.size() > e->ranges_cap) {
-        if (e->d_ranges) (void)hipFree(e->d_ranges);
-        e->ranges_cap = std::max<size_t>(all_ranges.size(), 2 * e->ranges_cap);
-        L2O_TRY(hipMalloc(&e->d_ranges, e->ranges_cap * sizeof(uint32_t)));
-    }
-    if (!all_ranges.empty())
-        L2O_TRY(hipMemcpyAsync(e->d_ranges, all_ranges.data(), all_ranges.size() * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, e->stream));
     if (all.size() > e->sitems_cap) {
         if (e->d_sitems) (void)hipFree(e->d_sitems);
         e->sitems_cap = std::max<size_t>(all.size(), 2 * e->sitems_cap);
@@ -647,7 +529,6 @@ int rebuild_lists(logo2opt_engine *e)
     }
     if (ai.size() > e->aitems_cap) {
         if (e->d_aitems) (void)hipFree(e->d_aitems);
-    if (e->d_ranges) (void)hipFree(e->d_ranges);
         e->aitems_cap = std::max<size_t>(ai.size(), 2 * e->aitems_cap);
         L2O_TRY(hipMalloc(&e->d_aitems, e->aitems_cap * sizeof(l2o_apply_item)));
     }
@@ -668,9 +549,6 @@ int rebuild_lists(logo2opt_engine *e)
 // the sweep launch of one group
 hipError_t launch_group_sweep(const logo2opt_engine *e, const logo2opt_engine::group &g)
 {
-    if (g.nblocks)
-        return l2o::launch_sweep_ranges(g.cfg, e->d_sitems + g.off, e->d_ranges + g.roff, g.nblocks, e->d_insts,
-                                        e->stream, g.prune);
     return l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune);
 }
 
@@ -962,7 +840,6 @@ int engine_create(int device, uint64_t budget, uint32_t max_slots, logo2opt_engi
     e->max_slots = max_slots;
     e->slots.resize(max_slots);
     if (const char *g = std::getenv("LOGO2OPT_GRAPHS")) e->graphs = std::strcmp(g, "0") != 0;
-    if (const char *r = std::getenv("LOGO2OPT_SWEEP_RANGES")) e->ranges_mode = std::atoi(r);
     bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&e->d_arena, budget) == hipSuccess &&
               hipMalloc(&e->d_insts, sizeof(l2o_inst) * max_slots) == hipSuccess &&
@@ -1079,7 +956,6 @@ int logo2opt_engine_destroy(logo2opt_engine *e)
     drop_graph(e);
     if (e->d_sitems) (void)hipFree(e->d_sitems);
     if (e->d_aitems) (void)hipFree(e->d_aitems);
-    if (e->d_ranges) (void)hipFree(e->d_ranges);
     if (e->d_arena) (void)hipFree(e->d_arena);
     if (e->d_insts) (void)hipFree(e->d_insts);
     if (e->d_live) (void)hipFree(e->d_live);

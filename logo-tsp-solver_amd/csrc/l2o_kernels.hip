@@ -236,12 +236,12 @@ __device__ __forceinline__ int wave_max_i32(int v)
 //
 // K = 8 needs 88 VGPRs unconstrained (5 waves per SIMD); asking for 6 waves
 // makes the allocator spill 68 bytes outside the hot loops instead
+#ifndef L2O_WPE6_MIN_K
+#define L2O_WPE6_MIN_K 8
+#endif
 //
-// sweep_item evaluates one work item; k_sweep runs one item per workgroup
-// (the hardware dispatcher balances the items), k_sweep_ranges runs a
-// contiguous range of items per workgroup on a grid sized to the resident
-// workgroups (the host splits the work into equal ranges, so every wave slot
-// stays busy to the end of the launch).
+// sweep_item evaluates one work item; k_sweep runs one item per workgroup and
+// the hardware dispatcher balances the items over the CUs.
 template <int K, int T, int WAVES, bool PRUNE>
 __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__restrict__ insts)
 {
@@ -599,21 +599,10 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
 }
 
 template <int K, int T, int WAVES, bool PRUNE>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >= 8 ? 6 : 1))) void k_sweep(
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >= L2O_WPE6_MIN_K ? 6 : 1))) void k_sweep(
     const l2o_sweep_item *__restrict__ items, l2o_inst *__restrict__ insts)
 {
     sweep_item<K, T, WAVES, PRUNE>(items[blockIdx.x], insts);
-}
-
-// items [ranges[b], ranges[b+1]) for workgroup b; the bounds are read with
-// scalar loads and kept in SGPRs, so the loop exit is uniform
-template <int K, int T, int WAVES, bool PRUNE>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >= 8 ? 6 : 1))) void k_sweep_ranges(
-    const l2o_sweep_item *__restrict__ items, const uint32_t *__restrict__ ranges, l2o_inst *__restrict__ insts)
-{
-    const uint32_t q0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ranges[blockIdx.x]);
-    const uint32_t q1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ranges[blockIdx.x + 1]);
-    for (uint32_t q = q0; q < q1; ++q) sweep_item<K, T, WAVES, PRUNE>(items[q], insts);
 }
 
 // ---------------------------------------------------------------------------
@@ -1024,45 +1013,6 @@ hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint3
     if (nitems == 0) return hipSuccess;
     L2O_SWEEP_SHAPES(L2O_SWEEP_CASE)
     return hipErrorInvalidValue;
-}
-
-#define L2O_SWEEP_RANGES_CASE(K_, T_, W_)                                                            \
-    if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \
-        if (prune)                                                                                   \
-            hipLaunchKernelGGL((k_sweep_ranges<K_, T_, W_, true>), dim3(nblocks), dim3(64 * W_), 0, stream, items, \
-                               ranges, insts);                                                       \
-        else                                                                                         \
-            hipLaunchKernelGGL((k_sweep_ranges<K_, T_, W_, false>), dim3(nblocks), dim3(64 * W_), 0, stream, items, \
-                               ranges, insts);                                                       \
-        return hipGetLastError();                                                                    \
-    }
-
-hipError_t launch_sweep_ranges(const sweep_cfg &cfg, const l2o_sweep_item *items, const uint32_t *ranges,
-                               uint32_t nblocks, l2o_inst *insts, hipStream_t stream, bool prune)
-{
-    if (nblocks == 0) return hipSuccess;
-    L2O_SWEEP_SHAPES(L2O_SWEEP_RANGES_CASE)
-    return hipErrorInvalidValue;
-}
-
-#define L2O_SWEEP_OCC_CASE(K_, T_, W_)                                                               \
-    if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \
-        if (prune)                                                                                   \
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_ranges<K_, T_, W_, true>, 64 * W_, 0); \
-        else                                                                                         \
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_ranges<K_, T_, W_, false>, 64 * W_, 0); \
-    }
-
-int sweep_ranges_resident(const sweep_cfg &cfg, bool prune)
-{
-    int dev = 0, cus = 0, per_cu = 0;
-    hipError_t e = hipErrorInvalidValue;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return 0;
-    L2O_SWEEP_SHAPES(L2O_SWEEP_OCC_CASE)
-    if (e != hipSuccess) return 0;
-    return per_cu * cus;
 }
 
 #define L2O_SWEEP_ROW(K_, T_, W_) {K_, T_, W_},

@@ -71,13 +71,6 @@ int sweep_shapes(int (*out)[3], int cap);
 // rows a bounding-box bound rules out (same results)
 hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint32_t nitems,
                         l2o_inst *insts, hipStream_t stream, bool prune = false);
-// workgroup b evaluates items [ranges[b], ranges[b+1]); the grid is sized
-// to the resident workgroups (sweep_ranges_resident) and the host balances
-// the ranges (same results as launch_sweep over the same items)
-hipError_t launch_sweep_ranges(const sweep_cfg &cfg, const l2o_sweep_item *items, const uint32_t *ranges,
-                               uint32_t nblocks, l2o_inst *insts, hipStream_t stream, bool prune = false);
-// workgroups of the ranges kernel that fit on the current device at once (0 on error)
-int sweep_ranges_resident(const sweep_cfg &cfg, bool prune);
 hipError_t launch_apply(const l2o_apply_item *items, uint32_t nitems, l2o_inst *insts, unsigned *live,
                         hipStream_t stream);
 hipError_t launch_ils(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStream_t stream);
