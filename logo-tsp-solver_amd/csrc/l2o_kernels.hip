@@ -234,10 +234,12 @@ __device__ __forceinline__ int wave_max_i32(int v)
 // the exact pass, and the selected move, are those of the full sweep.  After
 // a skipped run of blocks V is recomputed from the next block's first row.
 //
-// K = 8 needs 88 VGPRs unconstrained (5 waves per SIMD); asking for 6 waves
-// makes the allocator spill 68 bytes outside the hot loops instead
+// K = 6 and 8 need 86-90 VGPRs unconstrained (5 waves per SIMD); asking for
+// 6 waves makes the allocator keep 80 and spill 16-68 bytes per lane outside
+// the hot loops instead (the config-2 local search 1.7% faster, the batch
+// 1.1%, on one box A/B)
 #ifndef L2O_WPE6_MIN_K
-#define L2O_WPE6_MIN_K 8
+#define L2O_WPE6_MIN_K 6
 #endif
 //
 // sweep_item evaluates one work item; k_sweep runs one item per workgroup and
