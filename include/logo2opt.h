@@ -289,8 +289,9 @@ int logo2opt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_s
 int logo2opt_server_destroy(logo2opt_server *s);
 /* Queue a request (copied).  LOGO2OPT_EAGAIN (id = -1) when the queue is full.
  * route may be NULL: the server then builds the greedy nearest-neighbour tour
- * from city 0 (logo2opt_nn_tour) when it first tries to admit the request;
- * that time is part of the request's latency. */
+ * from city 0 (logo2opt_nn_tour) on a host thread of its own while the GPU
+ * runs the admitted requests; the request is admitted once its tour is
+ * built, so that time is part of its latency. */
 int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_coords *coords,
                            const uint32_t *route, const logo2opt_job *job, int64_t *id);
 /* Admit what fits, run up to `steps` batched steps (host check every

@@ -13,10 +13,14 @@
 // The reference has no serving layer; its closest relative is the multi-start
 // ILS driver with one solver per pthread (src/solvers/ILSGlobalSolverMT.cpp:86-195).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -33,6 +37,8 @@ double now_s()
 
 enum req_state { Q_QUEUED = 0, Q_RUNNING = 1, Q_DONE = 2, Q_REJECTED = 3 };
 
+struct build_job;
+
 struct request {
     int64_t id = 0;
     uint32_t n = 0;
@@ -43,31 +49,49 @@ struct request {
     int32_t slot = -1;
     int state = Q_QUEUED;
     int err = LOGO2OPT_OK;  // Q_REJECTED: why
-    bool construct = false;  // no start tour given: the server builds the NN tour
-    double t_construct = 0;  // seconds spent building it
+    double t_construct = 0;  // seconds the server spent building its start tour
+    // no start tour given: the server's builder thread makes the greedy NN
+    // tour while the GPU runs other requests; admission waits for it
+    std::shared_ptr<build_job> build;
     logo2opt_result res{};
 };
 
+// A start tour for the builder thread: inputs are fixed at submission, the
+// outputs are published by `ready` (release / acquire).
+struct build_job {
+    std::vector<logo2opt_city_coords> coords;
+    std::vector<uint32_t> route;
+    int rc = LOGO2OPT_OK;
+    double seconds = 0;
+    std::atomic<bool> ready{false};
+};
+
 // Greedy nearest-neighbour tour from city 0 (the serving workload's start)
-// over the request's coordinates, translated so the minimum is 0.  Invalid
-// coordinates are left to engine_add to reject.
-int construct_nn(request &r)
+// over the coordinates translated so the minimum is 0.  Invalid coordinates
+// are left to engine_add to reject.
+void build_nn(build_job &b)
 {
-    const uint32_t n = r.n;
-    double mnx = r.coords[0].x, mny = r.coords[0].y;
+    const double t0 = now_s();
+    const uint32_t n = (uint32_t)b.coords.size();
+    double mnx = b.coords[0].x, mny = b.coords[0].y;
     for (uint32_t k = 1; k < n; ++k) {
-        mnx = std::min(mnx, (double)r.coords[k].x);
-        mny = std::min(mny, (double)r.coords[k].y);
+        mnx = std::min(mnx, (double)b.coords[k].x);
+        mny = std::min(mny, (double)b.coords[k].y);
     }
     std::vector<int32_t> xy(2 * (size_t)n);
-    for (uint32_t k = 0; k < n; ++k) {
-        const double x = (double)r.coords[k].x - mnx, y = (double)r.coords[k].y - mny;
-        if (!(x >= 0 && x < 2147483648.0 && y >= 0 && y < 2147483648.0)) return LOGO2OPT_ECOORD;
+    b.rc = LOGO2OPT_OK;
+    for (uint32_t k = 0; k < n && b.rc == LOGO2OPT_OK; ++k) {
+        const double x = (double)b.coords[k].x - mnx, y = (double)b.coords[k].y - mny;
+        if (!(x >= 0 && x < 2147483648.0 && y >= 0 && y < 2147483648.0)) b.rc = LOGO2OPT_ECOORD;
         xy[2 * k] = (int32_t)x;
         xy[2 * k + 1] = (int32_t)y;
     }
-    r.route.resize(n);
-    return logo2opt_nn_tour(n, xy.data(), 0, r.route.data());
+    if (b.rc == LOGO2OPT_OK) {
+        b.route.resize(n);
+        b.rc = logo2opt_nn_tour(n, xy.data(), 0, b.route.data());
+    }
+    b.seconds = now_s() - t0;
+    b.ready.store(true, std::memory_order_release);
 }
 
 }  // namespace
@@ -88,6 +112,44 @@ struct logo2opt_server {
     std::map<int32_t, int64_t> slot_req;
     std::vector<int64_t> completed;  // not yet polled
     uint64_t n_submitted = 0, n_rejected = 0, n_admitted = 0, n_done = 0;
+    // builder thread for start tours (started with the first request that
+    // has none), FIFO like admission
+    std::thread builder;
+    std::mutex bmu;
+    std::condition_variable bcv;
+    std::deque<std::shared_ptr<build_job>> bq;
+    bool bstop = false;
+
+    void build_loop()
+    {
+        for (;;) {
+            std::shared_ptr<build_job> j;
+            {
+                std::unique_lock<std::mutex> lk(bmu);
+                bcv.wait(lk, [&] { return bstop || !bq.empty(); });
+                if (bstop) return;
+                j = bq.front();
+                bq.pop_front();
+            }
+            build_nn(*j);
+        }
+    }
+    void enqueue_build(std::shared_ptr<build_job> j)
+    {
+        std::lock_guard<std::mutex> lk(bmu);
+        if (!builder.joinable()) builder = std::thread([this] { build_loop(); });
+        bq.push_back(std::move(j));
+        bcv.notify_one();
+    }
+    ~logo2opt_server()
+    {
+        {
+            std::lock_guard<std::mutex> lk(bmu);
+            bstop = true;
+        }
+        bcv.notify_all();
+        if (builder.joinable()) builder.join();
+    }
 };
 
 extern "C" {
@@ -130,13 +192,19 @@ int logo2opt_server_submit(logo2opt_server *s, uint32_t n, const logo2opt_city_c
     r.id = s->next_id++;
     r.n = n;
     r.coords.assign(coords, coords + n);
-    if (route) r.route.assign(route, route + n);
-    else r.construct = true;
+    if (route) {
+        r.route.assign(route, route + n);
+    } else {
+        r.build = std::make_shared<build_job>();
+        r.build->coords = r.coords;
+    }
     if (job) r.job = *job;
     r.t_submit = now_s();
     *id = r.id;
     s->queue.push_back(r.id);
+    std::shared_ptr<build_job> j = r.build;
     s->reqs.emplace(r.id, std::move(r));
+    if (j) s->enqueue_build(std::move(j));
     return LOGO2OPT_OK;
 }
 
@@ -153,11 +221,12 @@ int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every
             break;
         int32_t slot = -1;
         int rc = LOGO2OPT_OK;
-        if (r.construct) {  // once, at the first admission attempt
-            const double t0 = now_s();
-            rc = construct_nn(r);
-            r.t_construct = now_s() - t0;
-            r.construct = false;
+        if (r.build) {
+            if (!r.build->ready.load(std::memory_order_acquire)) break;  // head's start tour not built yet
+            rc = r.build->rc;
+            r.route.swap(r.build->route);
+            r.t_construct = r.build->seconds;
+            r.build.reset();
         }
         if (rc == LOGO2OPT_OK) rc = logo2opt_engine_add(s->e, r.n, r.coords.data(), r.route.data(), &r.job, &slot);
         if (rc == LOGO2OPT_EAGAIN) break;  // head of line does not fit yet
@@ -350,6 +419,9 @@ int logo2opt_server_replay(logo2opt_server *s, uint32_t count, const double *arr
         if (running == 0 && queued == 0 && next < count) {
             const double wait = arrival_s[next] - (now_s() - t0);
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+        } else if (running == 0 && queued > 0) {
+            // the head of the queue waits for its start tour
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
     }
     if (elapsed_s) *elapsed_s = now_s() - t0;
