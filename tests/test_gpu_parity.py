@@ -71,7 +71,7 @@ def test_step_on_nn_tours_many_ties():
 
 @pytest.mark.parametrize("sync_every", [1, 64])
 def test_local_search_config1_move_sequence(sync_every):
-    # n=1000, coords U{0..9999}^2, random permutation start
+    # the 1,000-city workload: coords U{0..9999}^2, random permutation start
     n = 1000
     xy = O.gen_uniform(n, 10**4, 1)
     r0 = O.random_perm(n, 2)
@@ -246,10 +246,11 @@ def test_engine_admission_budget():
 
 
 def test_config2_full_size_local_optimum():
-    # 33,810 clustered cities in [0,10^6)^2, greedy NN start, to a
-    # local optimum.  Too long for the oracle's full move log; check the
-    # size-independent properties: valid permutation, exact length, the
-    # oracle finds no improving move, and the first moves agree.
+    # the 33,810 clustered cities in [0,10^6)^2, greedy NN start, to a
+    # local optimum: the size-independent properties (valid permutation,
+    # exact length, the oracle finds no improving move, the first moves
+    # agree).  test_gpu_full_size.py checks the whole move log and tour
+    # against the oracle's committed result.
     n = 33810
     xy = L.gen_clustered(n, 10**6, 2)
     r0 = L.nn_tour(xy, 0)
