@@ -375,6 +375,11 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
     in.kick_src = (uint32_t)id;
     in.est_scale = s.job.metric == l2o::L2O_ATT ? (float)(1.0 / std::sqrt(10.0)) : 1.0f;
     in.est_add = s.job.metric == l2o::L2O_EUC_2D ? 0.0f : 0.5f;
+    {
+        int32_t m = 0;
+        for (const int32_t v : s.xy) m = std::max(m, v);
+        in.qshift = l2o::tier0_shift((uint32_t)m);
+    }
     in.phase = phase;
     in.key = L2O_SENTINEL;
     // SplitMix64 -> xoshiro256** state (same seeding as the workload generators)

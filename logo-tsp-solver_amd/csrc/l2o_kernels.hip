@@ -14,7 +14,11 @@
 // returns the lexicographic minimum of (G, i, J) over G < 0, the move the
 // reference's strict-'<' scan keeps.
 //
-// Three filter tiers make every candidate cost four VALU operations:
+// Filter tiers, each sound on its own, make a candidate cost about two VALU
+// operations:
+//  0. Coarse bound on 8-bit coordinates (the full sweep): one v_sad_u8 per
+//     candidate decides whether a block of four rows runs tier 1 at all
+//     (see the Tier 0 notes before sweep_item).
 //  1. Integer L1 bound.  |dx|+|dy| <= sqrt(2) d and d < D + 1/2 (EUC_2D;
 //     for CEIL_2D d <= D, for ATT d <= sqrt(10) D), so G <= best-1 implies
 //         L1(a,b) + L1(a+1,b+1) <= ceil(sqrt(g)*best) + W[a+1] + W[b+1],
@@ -30,7 +34,8 @@
 //     |exact - estimate| <= slack (per instance, set by the host from the
 //     coordinate span), only for lanes flagged by tier 1.
 //  3. Exact FP64 sqrt + integer fix-up for the survivors (metric_dist).
-// Each tier only drops candidates with G > cap, where cap is the smaller of
+// Tiers 1-3 only drop candidates with G > cap (tier 0 only ones that tier 1
+// would drop as well), where cap is the smaller of
 // (the wave's best exact gain of EARLIER steps) - 1 -- a later step has a
 // larger i, so a tie with that best can never win -- and the best gain
 // published so far by any wave of the sweep (any i, so taken inclusively);
@@ -238,6 +243,22 @@ __device__ __forceinline__ int wave_max_i32(int v)
 // 6 waves makes the allocator keep 80 and spill 16-68 bytes per lane outside
 // the hot loops instead (the config-2 local search 1.7% faster, the batch
 // 1.1%, on one box A/B)
+// Tier 0 (full sweep).  With X = x >> q (2^q = Q), |x - x'| >= Q |X - X'| -
+// (Q - 1), so a candidate's tier-1 sum L1(a,b) + L1(a+1,b+1) - W[b+1] is at
+// least Q Sq - 4 (Q - 1) - W[b+1], where Sq is the same L1 sum of the shifted
+// coordinates (one v_sad_u8 of four bytes).  With wq = the lane's largest
+// ceil(W[b+1] / Q) and wa = ceil(W[a+1] / Q), a tier-1 hit has
+//   Sq - wq - wa <= floor((tb + 4 (Q - 1)) / Q),
+// so a block of steps in which no candidate meets this holds no tier-1 hit
+// and is skipped.  Each wave stages its rows' packed byte pairs and wa in LDS
+// once (one lane per row), and reads them back as broadcasts, so a step is
+// VALU work only: one subtract, six v_sad_u8 and three v_min3 for six
+// candidates.  Sq <= 1,020 and wq, wa < 2^10; the sum carries a bias of 2^16
+// so that it stays positive.
+#ifndef L2O_TIER0
+#define L2O_TIER0 1
+#endif
+#define L2O_T0_BIAS 0x10000u
 #ifndef L2O_WPE6_MIN_K
 #define L2O_WPE6_MIN_K 6
 #endif
@@ -248,6 +269,7 @@ template <int K, int T, int WAVES, bool PRUNE>
 __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__restrict__ insts)
 {
     constexpr int UF = 4;  // steps per unrolled iteration
+    constexpr bool TIER0 = !PRUNE && L2O_TIER0;
     static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
     static_assert(!PRUNE || T <= 64 * UF, "pruning tests one block of rows per lane");
     l2o_inst *in = insts + it.inst;
@@ -261,10 +283,11 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     const float slack = ci->slack;
     const uint32_t metric = ci->metric;
     const float es = ci->est_scale, ea = ci->est_add;
+    const uint32_t qs = ci->qshift;
     l2o_pt *const p0 = ci->pts[0], *const p1 = ci->pts[1];
     // (an empty asm that consumes them keeps the loads above the branches)
     asm volatile("" ::"s"(phase), "s"(par), "s"(n), "s"(slack), "s"(p0), "s"(p1), "s"(it.a0), "s"(it.b0),
-                 "s"(it.rows), "s"(metric), "s"(es), "s"(ea));
+                 "s"(it.rows), "s"(metric), "s"(es), "s"(ea), "s"(qs));
     if (phase != L2O_LO) return;
     const gpt *__restrict__ P = (const gpt *)(par ? p1 : p0);
     const int a0 = (int)it.a0;
@@ -300,6 +323,7 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     // (W first, coordinates after the offsets: fewer registers live at once)
     uint32_t cx[K + 1], cy[K + 1], o[K + 1];
     int wmax = 0;  // PRUNE: the lane's, then the wave's largest W of a column b+1
+    int wq = 0;    // TIER0: the lane's largest ceil(W / 2^qs) of a column b+1
     {
         uint32_t Wn[K + 1];
 #pragma unroll
@@ -309,6 +333,10 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         if constexpr (PRUNE) {
 #pragma unroll
             for (int j = 1; j <= K; ++j) wmax = max(wmax, (int)Wn[j]);
+        }
+        if constexpr (TIER0) {
+#pragma unroll
+            for (int j = 1; j <= K; ++j) wq = max(wq, (max((int)Wn[j], 0) + (1 << qs) - 1) >> qs);
         }
         // The chain o_i = (-1)^i (o_0 - A_i), A_i = sum_{t=1..i} (-1)^t W_t
         // over the wave's columns i = lK + j: A at each lane's first column is
@@ -355,6 +383,31 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         by1 = wave_max_i32(My);
         wmax = wave_max_i32(wmax);
     }
+    // TIER0: column pairs (b, b+1) of the lane as bytes of coordinates >> qs,
+    // and the wave's rows in LDS: t0rp[t] = the same bytes of rows (a, a+1),
+    // t0wa[t] = ceil(W[a+1] / 2^qs), a = a0+t, for the rows of whole blocks
+    // (a+1 <= a0 + t_end <= n-3).  Only this wave reads its rows, and LDS
+    // operations of one wave complete in order: no barrier.
+    uint32_t cpk[TIER0 ? K : 1];
+    uint32_t c0 = 0;
+    __shared__ __attribute__((aligned(16))) uint32_t t0lds[TIER0 ? 2 * WAVES * T : 1];
+    uint32_t *const t0rp = t0lds + (TIER0 ? 2 * w * T : 0);
+    uint32_t *const t0wa = t0rp + (TIER0 ? T : 0);
+    if constexpr (TIER0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            cpk[k] = (cx[k] >> qs) | ((cy[k] >> qs) << 8) | ((cx[k + 1] >> qs) << 16) | ((cy[k + 1] >> qs) << 24);
+        c0 = L2O_T0_BIAS - (uint32_t)wq;
+        const int rows0 = max(t_end, 0) / UF * UF;
+        for (int t = lane; t < rows0; t += 64) {
+            const uint32_t xa = (uint32_t)P[a0 + t].x, ya = (uint32_t)P[a0 + t].y;
+            const uint32_t xb = (uint32_t)P[a0 + t + 1].x, yb = (uint32_t)P[a0 + t + 1].y;
+            const uint32_t wb = (uint32_t)P[a0 + t + 1].w;
+            t0rp[t] = (xa >> qs) | ((ya >> qs) << 8) | ((xb >> qs) << 16) | ((yb >> qs) << 24);
+            t0wa[t] = (wb + (1u << qs) - 1) >> qs;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
     // diagonal waves: candidate (a0+t, bl+k) exists iff t - lim0 <= k
     const int lim0 = bl - 2 - a0;
 
@@ -380,10 +433,16 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     const int g = tier1_g(metric);
     int tb = 0;                // tier 1: ceil(sqrt(g) * (cap + 1))
     float thr = slack - 1.0f;  // tier 2: cap + slack
+    // TIER0: BIAS + floor((tb + 4 (2^qs - 1)) / 2^qs)
+    auto t0_of = [&](int b) __attribute__((always_inline)) {
+        return (int)L2O_T0_BIAS + ((b + 4 * ((1 << qs) - 1)) >> qs);
+    };
+    int t0h = t0_of(0);
     auto set_cap = [&](int c) __attribute__((always_inline)) {
         cap = c;
         tb = __builtin_amdgcn_readfirstlane(tier1_base(c + 1, g));
         thr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)c + slack)));
+        if constexpr (TIER0) t0h = t0_of(tb);
     };
     // (PRUNE: read after the row test below, by waves with a live block only;
     // the row test then uses cap = -1, a valid, looser threshold)
@@ -537,6 +596,56 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
             if (block(t, r1, std::integral_constant<int, 1>{}, mask_tag)) slow(t, 1);
         }
     };
+    // TIER0: every block of UF steps first gets the coarse test of every
+    // candidate; only a block in which some candidate passes it loads its
+    // rows, runs tier 1 (block) and, on a tier-1 hit, the slow path.  V is
+    // carried from block to block while they run and recomputed from the
+    // block's first row after a skipped run.
+    auto run_tier0 = [&](auto mask_tag) __attribute__((always_inline)) {
+        using UFc = std::integral_constant<int, UF>;
+        using UF1c = std::integral_constant<int, UF + 1>;
+        const int nmain = t_end / UF;  // whole blocks
+        int tv = 0;                    // V holds row a0+tv
+        const uint4 *rp4 = (const uint4 *)t0rp, *wa4 = (const uint4 *)t0wa;
+        auto process = [&](int q, const uint4 rp, const uint4 wa) __attribute__((always_inline)) {
+            int m = 0x7FFFFFFF;
+            const uint32_t rs[UF] = {rp.x, rp.y, rp.z, rp.w}, ws[UF] = {wa.x, wa.y, wa.z, wa.w};
+#pragma unroll
+            for (int s = 0; s < UF; ++s) {
+                const uint32_t c = c0 - ws[s];
+#pragma unroll
+                for (int k = 0; k < K; ++k) m = min(m, (int)__builtin_amdgcn_sad_u8(cpk[k], rs[s], c));
+            }
+            if (!__ballot(m <= t0h)) return;
+            const int t = q * UF;
+            int r[UF + 1][3];
+            load_rows(t - 1, r, UF1c{});  // rows t .. t+UF
+            if (t != tv) {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    V[k] = sad((uint32_t)r[0][1], cy[k], sad((uint32_t)r[0][0], cx[k], o[k]));
+            }
+            if (block(t, r + 1, UFc{}, mask_tag)) slow(t, UF);
+            tv = t + UF;
+        };
+        // (no software prefetch: a second buffer of LDS values spills the
+        // column registers; other waves on the SIMD cover the LDS latency)
+        for (int q = 0; q < nmain; ++q) process(q, rp4[q], wa4[q]);
+        // the rows after the last whole block: tier 1 directly
+        int t = nmain * UF;
+        if (t < t_end) {
+            if (t != tv) {
+                const uint32_t x = (uint32_t)prow[4 * t], y = (uint32_t)prow[4 * t + 1];
+#pragma unroll
+                for (int k = 0; k < K; ++k) V[k] = sad(y, cy[k], sad(x, cx[k], o[k]));
+            }
+            for (; t < t_end; ++t) {
+                int r1[1][3];
+                load_rows(t, r1, std::integral_constant<int, 1>{});
+                if (block(t, r1, std::integral_constant<int, 1>{}, mask_tag)) slow(t, 1);
+            }
+        }
+    };
     // PRUNE: live blocks only.  Each buffer holds a block's first row and its
     // four step rows; the next live block's rows are requested before this
     // one computes.
@@ -590,6 +699,9 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     if constexpr (PRUNE) {
         if (diag) run_pruned(std::true_type{});
         else run_pruned(std::false_type{});
+    } else if constexpr (TIER0) {
+        if (diag) run_tier0(std::true_type{});
+        else run_tier0(std::false_type{});
     } else {
         if (diag) run(std::true_type{});
         else run(std::false_type{});

@@ -53,6 +53,16 @@ __host__ __device__ inline int32_t ceil_sqrtg(int32_t d, int32_t g)
 
 __host__ __device__ inline int32_t ceil_sqrt2(int32_t d) { return ceil_sqrtg(d, 2); }
 
+// The sweep's coarse test (tier 0) works on coordinates >> qshift, which must
+// fit in 8 bits: the smallest shift that brings the largest translated
+// coordinate (below 2^22) under 256.
+__host__ __device__ inline uint32_t tier0_shift(uint32_t max_coord)
+{
+    uint32_t s = 0;
+    while ((max_coord >> s) > 255u) ++s;
+    return s;
+}
+
 // Sweep tile shape: each workgroup has `waves` wave64s, each lane owns `k`
 // adjacent candidate columns for `t` rows (the last lane of a wave one fewer:
 // its last column is the next wave's first).  A workgroup covers

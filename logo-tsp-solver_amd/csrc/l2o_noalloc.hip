@@ -170,6 +170,7 @@ extern "C" int logo2opt_step_noalloc(int device, uint32_t n, const logo2opt_city
     in.group_lead = L2O_NO_GROUP;
     in.est_scale = metric == LOGO2OPT_METRIC_ATT ? (float)(1.0 / std::sqrt(10.0)) : 1.0f;
     in.est_add = metric == LOGO2OPT_METRIC_EUC_2D ? 0.0f : 0.5f;
+    in.qshift = l2o::tier0_shift((uint32_t)std::fmax(mxx - mnx, mxy - mny));
     in.phase = L2O_LO;
     in.key = L2O_SENTINEL;
 

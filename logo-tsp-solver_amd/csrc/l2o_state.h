@@ -69,6 +69,7 @@ struct l2o_inst {
     uint32_t kick_src;    // slot whose best tour the next kick starts from
     float est_scale;      // FP32 tier: D ~ sqrt(s) * est_scale + est_add
     float est_add;
+    uint32_t qshift;      // tier 0: coordinates >> qshift fit in 8 bits
     // ---- dynamic
     uint32_t phase;
     uint32_t parity;

@@ -133,3 +133,138 @@ def test_tier1_extreme_geometry():
     xy = O.xy_of(x, y)
     for s in range(6):
         _check(xy, O.random_perm(len(x), s))
+
+
+# --- tier 0: the coarse test on coordinates >> q that gates tier 1 ----------
+
+T0_BIAS = 1 << 16
+
+
+def tier0_shift(max_coord):
+    """The host's shift (l2o_launch.h tier0_shift): smallest q with
+    max_coord >> q <= 255."""
+    q = 0
+    while (max_coord >> q) > 255:
+        q += 1
+    return q
+
+
+def _tier0_keeps(Sq, wq, tb, wa, q):
+    """The kernel's test, with wq = the column's (or the lane's) ceil(W[b+1]/Q)
+    and wa = W[a+1]: BIAS + Sq - wq - ceil(wa/Q) <= BIAS + floor((tb + 4 (Q -
+    1)) / Q), floor by arithmetic shift (numpy arrays or ints)."""
+    Q = 1 << q
+    return T0_BIAS + Sq - wq - (-(-wa // Q)) <= T0_BIAS + ((tb + 4 * (Q - 1)) >> q)
+
+
+def _tier0_cands(X, Y, q):
+    """Per candidate (a, b), a+2 <= b <= n-2, of the tour order (X, Y): the
+    tier-1 sum L1(a,b) + L1(a+1,b+1), W[a+1], W[b+1] and the L1 sum Sq of the
+    shifted coordinates, as int64 arrays."""
+    n = len(X)
+    X = np.asarray(X, np.int64)
+    Y = np.asarray(Y, np.int64)
+    dx = X - np.roll(X, 1)
+    dy = Y - np.roll(Y, 1)
+    D = [(math.isqrt(4 * int(u * u + v * v)) + 1) // 2 for u, v in zip(dx, dy)]
+    W = np.array([ceil_sqrt2_kernel(d) for d in D], np.int64)
+    a, b = np.array([(a, b) for a in range(n - 3) for b in range(a + 2, n - 1)]).T
+    L = np.abs(X[a] - X[b]) + np.abs(Y[a] - Y[b]) + np.abs(X[a + 1] - X[b + 1]) + np.abs(Y[a + 1] - Y[b + 1])
+    Xs, Ys = X >> q, Y >> q
+    Sq = (np.abs(Xs[a] - Xs[b]) + np.abs(Ys[a] - Ys[b]) + np.abs(Xs[a + 1] - Xs[b + 1]) +
+          np.abs(Ys[a + 1] - Ys[b + 1]))
+    return L, W[a + 1], W[b + 1], Sq, b
+
+
+def _check_tier0(X, Y, lane_cols=6):
+    """Every candidate that passes tier 1 (L - W[b+1] <= tb + W[a+1]) at any
+    threshold passes tier 0, with wq = ceil(W[b+1]/2^q) and with the lane's
+    largest such value (columns grouped lane_cols to a lane)."""
+    q = tier0_shift(int(max(np.max(X), np.max(Y))))
+    assert max(np.max(X), np.max(Y)) >> q <= 255
+    L, wa, wb, Sq, b = _tier0_cands(X, Y, q)
+    Q = 1 << q
+    wq = -(-wb // Q)
+    lane = (b - 2) // lane_cols
+    wq_lane = np.zeros(lane.max() + 1, np.int64)
+    np.maximum.at(wq_lane, lane, wq)
+    slack = L - wb - wa  # tier 1 passes iff slack <= tb
+    for tb in sorted(set(int(v) for v in np.unique(slack)[::max(1, len(slack) // 40)]) | {0, int(slack.min())}):
+        if tb > 0:
+            continue
+        t1 = slack <= tb
+        assert np.all(_tier0_keeps(Sq, wq, tb, wa, q)[t1]), (q, tb)
+        assert np.all(_tier0_keeps(Sq, wq_lane[lane], tb, wa, q)[t1]), (q, tb)
+        if q == 0:  # no shift: tier 0 is tier 1
+            assert np.array_equal(_tier0_keeps(Sq, wq, tb, wa, q), t1)
+    return q
+
+
+def test_tier0_shift():
+    assert [tier0_shift(m) for m in (0, 1, 255, 256, 511, 512, (1 << 22) - 1)] == [0, 0, 0, 1, 1, 2, 14]
+
+
+@pytest.mark.parametrize("n,side,kind,seed", [
+    (70, 200, "u", 1), (80, 10**4, "c", 2), (70, (1 << 22) - 1, "u", 3), (90, 3000, "u", 4), (60, 2**16, "c", 5),
+])
+def test_tier0_never_drops_a_tier1_hit(n, side, kind, seed):
+    xy = O.gen_uniform(n, side, seed) if kind == "u" else O.gen_clustered(n, side, seed)
+    x, y = xy[0::2].astype(np.int64), xy[1::2].astype(np.int64)
+    x -= x.min()
+    y -= y.min()
+    for s in range(2):
+        r = O.random_perm(n, 200 + s)
+        _check_tier0(x[r], y[r])
+    r = O.local_search(xy, O.nn_tour(xy, 0), max_moves=n // 8)[0]
+    _check_tier0(x[r], y[r])
+
+
+def _tier0_one(P, q):
+    """(L, W[a+1], W[b+1], Sq) of candidate (0, 3) of the 6-point tour P."""
+    X = np.array([p[0] for p in P], np.int64)
+    Y = np.array([p[1] for p in P], np.int64)
+    L, wa, wb, Sq, b = _tier0_cands(X, Y, q)
+    pairs = [(a, bb) for a in range(3) for bb in range(a + 2, 5)]
+    k = pairs.index((0, 3))
+    return int(L[k]), int(wa[k]), int(wb[k]), int(Sq[k])
+
+
+def test_tier0_tight_instances():
+    # Candidate (0, 3) of a 6-city tour, Q = 16.  Each coordinate of P0 is
+    # Q - 1 mod Q and the matching one of P3 is 0 mod Q, and likewise P1 and
+    # P4, so each of the four differences loses exactly Q - 1 to the shift:
+    # L = Q Sq - 4 (Q - 1).  tb is set so that tier 1 passes with no margin.
+    q, Q = 4, 16
+    fill = [(0, 240), (240, 0)]
+
+    def case(P):
+        L, wa, wb, Sq = _tier0_one(P, q)
+        assert L == Q * Sq - 4 * (Q - 1)
+        tb = L - wa - wb
+        assert tb <= 0
+        return L, wa, wb, Sq, tb
+    # (1) d(P0,P1) = d(P3,P4) = 181, W = 256 = 16 Q: tier 0 passes with no
+    # margin either, so a rounding term dropped from its threshold or W[a+1]
+    # left out of it fails
+    L, wa, wb, Sq, tb = case([(63, 95), (191, 223), fill[0], (64, 96), (192, 224), fill[1]])
+    assert (L, wa, wb, Sq) == (4, 256, 256, 4)
+    wq = -(-wb // Q)
+    assert _tier0_keeps(Sq, wq, tb, wa, q)
+    assert Sq - wq - wa // Q == (tb + 4 * (Q - 1)) >> q  # margin 0
+    assert not Sq - wq - wa // Q <= (tb + 3 * (Q - 1)) >> q  # 3 (Q-1) for 4 (Q-1)
+    assert not Sq - wq <= (tb + 4 * (Q - 1)) >> q  # W[a+1] dropped
+    # (2) W[a+1] = 160 = 10 Q, W[b+1] = 177 = 11 Q + 1: ceil(W[b+1]/Q) keeps
+    # the candidate, floor does not
+    L, wa, wb, Sq, tb = case([(63, 95), (143, 175), fill[0], (64, 96), (144, 192), fill[1]])
+    assert (wa, wb) == (160, 177)
+    assert _tier0_keeps(Sq, -(-wb // Q), tb, wa, q)
+    assert not _tier0_keeps(Sq, wb // Q, tb, wa, q)
+    # (3) W[a+1] = 132 = 8 Q + 4, W[b+1] = 160: ceil(W[a+1]/Q) keeps it, floor
+    # does not
+    L, wa, wb, Sq, tb = case([(63, 95), (143, 143), fill[0], (64, 96), (144, 176), fill[1]])
+    assert (wa, wb) == (132, 160)
+    assert _tier0_keeps(Sq, wb // Q, tb, wa, q)
+    assert not Sq - wb // Q - wa // Q <= (tb + 4 * (Q - 1)) >> q
+    # tier 0 keeps a candidate at every looser threshold too
+    for t in range(tb, 1):
+        assert _tier0_keeps(Sq, wb // Q, t, wa, q)
