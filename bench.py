@@ -75,7 +75,7 @@ def sweep_kernel_name(n, table=None):
     """Name of the sweep kernel instance the library launches for size n."""
     rows = L.tuning_get(L.TABLE_SINGLE if table is None else table)
     k, t, w = next((sh for mx, sh in rows if n <= mx), rows[-1][1])
-    return f"l2o::k_sweep<{k},{t},{w},false>"
+    return f"l2o::k_sweep<{k},{t},{w},0>"  # mode 0: the full sweep
 
 
 def config2(seed=2):

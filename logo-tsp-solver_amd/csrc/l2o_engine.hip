@@ -230,7 +230,7 @@ struct logo2opt_engine {
     struct group {
         sweep_cfg cfg;
         uint32_t off, count, id;
-        bool prune;  // launched as the row-pruning variant
+        int mode;  // sweep variant (L2O_SWEEP_*)
     };
     std::vector<group> groups;
     l2o_sweep_item *d_sitems = nullptr;
@@ -423,6 +423,12 @@ void drop_graph(logo2opt_engine *e)
 // it the pruned sweep measured 4-9% slower (per-wave fixed costs dominate),
 // from 16,000 on it wins (1.04x there, 1.2x at 24,000, 1.6x at 33,810)
 #define L2O_PRUNE_AUTO_MIN_N 16000u
+// Pruned instances below this many cities send their live blocks through
+// tier 0; larger ones, whose live blocks lie close to the columns and mostly
+// pass it, go straight to tier 1 (measured: 33,810 and 100,000 cities 3-5%
+// faster without tier 0, the 2,048-instance batch of 100-5,000 cities 7%
+// faster with it)
+#define L2O_PRUNE_T0_MAX_N 16000u
 
 // 1/L2O_TAIL_SPLIT of a launch's work, at its end, goes in items of
 // 1/L2O_TAIL_DIV of the shape's rows (T/L2O_TAIL_DIV must stay a multiple of 4)
@@ -451,7 +457,7 @@ uint64_t item_work(const l2o_sweep_item &it, const sweep_cfg &c, uint32_t n)
 
 int rebuild_lists(logo2opt_engine *e)
 {
-    std::map<std::tuple<int, int, int, bool>, std::vector<l2o_sweep_item>> by_cfg;
+    std::map<std::tuple<int, int, int, int>, std::vector<l2o_sweep_item>> by_cfg;
     std::vector<l2o_apply_item> ai;
     e->ils_grid = 0;
     e->any_ils = false;
@@ -469,7 +475,8 @@ int rebuild_lists(logo2opt_engine *e)
         const uint32_t n = s.n;
         const sweep_cfg c = cfg_for(n, table);
         const bool pr = e->prune == LOGO2OPT_PRUNE_ALL || (e->prune == LOGO2OPT_PRUNE_AUTO && n >= L2O_PRUNE_AUTO_MIN_N);
-        auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves, pr)];
+        const int mode = !pr ? l2o::L2O_SWEEP_FULL : n < L2O_PRUNE_T0_MAX_N ? l2o::L2O_SWEEP_PRUNE : l2o::L2O_SWEEP_PRUNE_T1;
+        auto &v = by_cfg[std::make_tuple(c.k, c.t, c.waves, mode)];
         // rows a <= n-4; the first tile of a row block starts at the
         // diagonal, b0 = a0 + 2
         for (uint32_t a0 = 0; a0 + 4 <= n; a0 += c.t)
@@ -486,7 +493,7 @@ int rebuild_lists(logo2opt_engine *e)
     e->groups.clear();
     for (auto &kv : by_cfg) {
         const sweep_cfg c{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
-        const bool pr = std::get<3>(kv.first);
+        const int mode = std::get<3>(kv.first);
         // Items are dispatched in list order (largest instances first), so at
         // any moment the device works on a window of consecutive items of one
         // or a few instances, whose points stay hot in every XCD's L2.
@@ -524,7 +531,7 @@ int rebuild_lists(logo2opt_engine *e)
         v.swap(sorted);
         const uint32_t cnt = (uint32_t)v.size();
         if (e->groups.size() >= L2O_MAX_GROUPS) return LOGO2OPT_EINVAL;
-        e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size(), pr});
+        e->groups.push_back({c, (uint32_t)all.size(), cnt, (uint32_t)e->groups.size(), mode});
         all.insert(all.end(), v.begin(), v.end());
     }
     if (all.size() > e->sitems_cap) {
@@ -554,7 +561,7 @@ int rebuild_lists(logo2opt_engine *e)
 // the sweep launch of one group
 hipError_t launch_group_sweep(const logo2opt_engine *e, const logo2opt_engine::group &g)
 {
-    return l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.prune);
+    return l2o::launch_sweep(g.cfg, e->d_sitems + g.off, g.count, e->d_insts, e->stream, g.mode);
 }
 
 int issue_step(logo2opt_engine *e)

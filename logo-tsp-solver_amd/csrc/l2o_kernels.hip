@@ -16,9 +16,9 @@
 //
 // Filter tiers, each sound on its own, make a candidate cost about two VALU
 // operations:
-//  0. Coarse bound on 8-bit coordinates (the full sweep): one v_sad_u8 per
-//     candidate decides whether a block of four rows runs tier 1 at all
-//     (see the Tier 0 notes before sweep_item).
+//  0. Coarse bound on 8-bit coordinates: one v_sad_u8 per candidate decides
+//     whether a block of four rows runs tier 1 at all (see the Tier 0 notes
+//     before sweep_item).
 //  1. Integer L1 bound.  |dx|+|dy| <= sqrt(2) d and d < D + 1/2 (EUC_2D;
 //     for CEIL_2D d <= D, for ATT d <= sqrt(10) D), so G <= best-1 implies
 //         L1(a,b) + L1(a+1,b+1) <= ceil(sqrt(g)*best) + W[a+1] + W[b+1],
@@ -243,7 +243,8 @@ __device__ __forceinline__ int wave_max_i32(int v)
 // 6 waves makes the allocator keep 80 and spill 16-68 bytes per lane outside
 // the hot loops instead (the config-2 local search 1.7% faster, the batch
 // 1.1%, on one box A/B)
-// Tier 0 (full sweep).  With X = x >> q (2^q = Q), |x - x'| >= Q |X - X'| -
+// Tier 0 (the full sweep, and the live blocks of the L2O_SWEEP_PRUNE
+// variant).  With X = x >> q (2^q = Q), |x - x'| >= Q |X - X'| -
 // (Q - 1), so a candidate's tier-1 sum L1(a,b) + L1(a+1,b+1) - W[b+1] is at
 // least Q Sq - 4 (Q - 1) - W[b+1], where Sq is the same L1 sum of the shifted
 // coordinates (one v_sad_u8 of four bytes).  With wq = the lane's largest
@@ -265,13 +266,14 @@ __device__ __forceinline__ int wave_max_i32(int v)
 //
 // sweep_item evaluates one work item; k_sweep runs one item per workgroup and
 // the hardware dispatcher balances the items over the CUs.
-template <int K, int T, int WAVES, bool PRUNE>
+template <int K, int T, int WAVES, int MODE>
 __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__restrict__ insts)
 {
     constexpr int UF = 4;  // steps per unrolled iteration
-    constexpr bool TIER0 = !PRUNE && L2O_TIER0;
+    constexpr bool PRUNE = MODE != L2O_SWEEP_FULL;
+    constexpr bool TIER0 = L2O_TIER0 && MODE != L2O_SWEEP_PRUNE_T1;
     static_assert(T % UF == 0, "T must be a multiple of the unroll factor");
-    static_assert(!PRUNE || T <= 64 * UF, "pruning tests one block of rows per lane");
+    static_assert(!(PRUNE || TIER0) || T <= 64 * UF, "pruning and tier 0 keep one bit per block of rows");
     l2o_inst *in = insts + it.inst;
     // The fields this kernel reads but no wave writes come in one batch of
     // scalar loads before any branch (constant address space: other kernels
@@ -393,20 +395,23 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     __shared__ __attribute__((aligned(16))) uint32_t t0lds[TIER0 ? 2 * WAVES * T : 1];
     uint32_t *const t0rp = t0lds + (TIER0 ? 2 * w * T : 0);
     uint32_t *const t0wa = t0rp + (TIER0 ? T : 0);
+    auto t0_pack = [&](uint32_t xa, uint32_t ya, uint32_t xb, uint32_t yb) __attribute__((always_inline)) {
+        return (xa >> qs) | ((ya >> qs) << 8) | ((xb >> qs) << 16) | ((yb >> qs) << 24);
+    };
+    auto t0_wq = [&](uint32_t wb) __attribute__((always_inline)) { return (wb + (1u << qs) - 1) >> qs; };
     if constexpr (TIER0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            cpk[k] = (cx[k] >> qs) | ((cy[k] >> qs) << 8) | ((cx[k + 1] >> qs) << 16) | ((cy[k + 1] >> qs) << 24);
+        for (int k = 0; k < K; ++k) cpk[k] = t0_pack(cx[k], cy[k], cx[k + 1], cy[k + 1]);
         c0 = L2O_T0_BIAS - (uint32_t)wq;
-        const int rows0 = max(t_end, 0) / UF * UF;
-        for (int t = lane; t < rows0; t += 64) {
-            const uint32_t xa = (uint32_t)P[a0 + t].x, ya = (uint32_t)P[a0 + t].y;
-            const uint32_t xb = (uint32_t)P[a0 + t + 1].x, yb = (uint32_t)P[a0 + t + 1].y;
-            const uint32_t wb = (uint32_t)P[a0 + t + 1].w;
-            t0rp[t] = (xa >> qs) | ((ya >> qs) << 8) | ((xb >> qs) << 16) | ((yb >> qs) << 24);
-            t0wa[t] = (wb + (1u << qs) - 1) >> qs;
+        if constexpr (!PRUNE) {
+            // one lane per row (PRUNE: from the row test's points, below)
+            const int rows0 = max(t_end, 0) / UF * UF;
+            for (int t = lane; t < rows0; t += 64) {
+                t0rp[t] = t0_pack(P[a0 + t].x, P[a0 + t].y, P[a0 + t + 1].x, P[a0 + t + 1].y);
+                t0wa[t] = t0_wq(P[a0 + t + 1].w);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
     // diagonal waves: candidate (a0+t, bl+k) exists iff t - lim0 <= k
     const int lim0 = bl - 2 - a0;
@@ -468,6 +473,18 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         }
         live = __ballot(keep);
         if (!live) return;  // nothing to evaluate, nothing published
+        if constexpr (TIER0) {
+            // lane l holds the points of block l (rows 4l .. 4l+4), which are
+            // unclamped for a whole block
+            if (keep && r0 + UF <= t_end) {
+#pragma unroll
+                for (int s = 0; s < UF; ++s) {
+                    t0rp[r0 + s] = t0_pack(qx[s], qy[s], qx[s + 1], qy[s + 1]);
+                    t0wa[r0 + s] = t0_wq(qw[s + 1]);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
         const unsigned long long g = key_now();
         if (g != L2O_SENTINEL && key_gain(g) < cap) set_cap(key_gain(g));
     }
@@ -604,7 +621,7 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     auto run_tier0 = [&](auto mask_tag) __attribute__((always_inline)) {
         using UFc = std::integral_constant<int, UF>;
         using UF1c = std::integral_constant<int, UF + 1>;
-        const int nmain = t_end / UF;  // whole blocks
+        const int nmain = max(t_end, 0) / UF;  // whole blocks
         int tv = 0;                    // V holds row a0+tv
         const uint4 *rp4 = (const uint4 *)t0rp, *wa4 = (const uint4 *)t0wa;
         auto process = [&](int q, const uint4 rp, const uint4 wa) __attribute__((always_inline)) {
@@ -630,10 +647,21 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         };
         // (no software prefetch: a second buffer of LDS values spills the
         // column registers; other waves on the SIMD cover the LDS latency)
-        for (int q = 0; q < nmain; ++q) process(q, rp4[q], wa4[q]);
-        // the rows after the last whole block: tier 1 directly
+        if constexpr (PRUNE) {
+            // live blocks only
+            unsigned long long todo = live & (nmain >= 64 ? ~0ull : (1ull << nmain) - 1);
+            while (todo) {
+                const int q = __builtin_ctzll(todo);
+                todo &= todo - 1;
+                process(q, rp4[q], wa4[q]);
+            }
+        } else {
+            for (int q = 0; q < nmain; ++q) process(q, rp4[q], wa4[q]);
+        }
+        // the rows after the last whole block (PRUNE: if the bound left any):
+        // tier 1 directly
         int t = nmain * UF;
-        if (t < t_end) {
+        if (t < t_end && (!PRUNE || ((live >> nmain) & 1))) {
             if (t != tv) {
                 const uint32_t x = (uint32_t)prow[4 * t], y = (uint32_t)prow[4 * t + 1];
 #pragma unroll
@@ -652,7 +680,7 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     auto run_pruned = [&](auto mask_tag) __attribute__((always_inline)) {
         using UFc = std::integral_constant<int, UF>;
         using UF1c = std::integral_constant<int, UF + 1>;
-        const int nmain = t_end / UF;  // whole blocks
+        const int nmain = max(t_end, 0) / UF;  // whole blocks
         unsigned long long todo = live & (nmain >= 64 ? ~0ull : (1ull << nmain) - 1);
         int tv = 0;  // V holds row a0+tv
         auto reseat = [&](int t, int x, int y) __attribute__((always_inline)) {
@@ -696,12 +724,12 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
             }
         }
     };
-    if constexpr (PRUNE) {
-        if (diag) run_pruned(std::true_type{});
-        else run_pruned(std::false_type{});
-    } else if constexpr (TIER0) {
+    if constexpr (TIER0) {
         if (diag) run_tier0(std::true_type{});
         else run_tier0(std::false_type{});
+    } else if constexpr (PRUNE) {
+        if (diag) run_pruned(std::true_type{});
+        else run_pruned(std::false_type{});
     } else {
         if (diag) run(std::true_type{});
         else run(std::false_type{});
@@ -712,11 +740,11 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     if (lane == 0 && wfb) atomicAdd(&in->fallbacks, (unsigned long long)wfb);
 }
 
-template <int K, int T, int WAVES, bool PRUNE>
+template <int K, int T, int WAVES, int MODE>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(K >= L2O_WPE6_MIN_K ? 6 : 1))) void k_sweep(
     const l2o_sweep_item *__restrict__ items, l2o_inst *__restrict__ insts)
 {
-    sweep_item<K, T, WAVES, PRUNE>(items[blockIdx.x], insts);
+    sweep_item<K, T, WAVES, MODE>(items[blockIdx.x], insts);
 }
 
 // ---------------------------------------------------------------------------
@@ -1114,17 +1142,20 @@ namespace l2o {
 
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \
-        if (prune)                                                                                   \
-            hipLaunchKernelGGL((k_sweep<K_, T_, W_, true>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
+        if (mode == L2O_SWEEP_PRUNE)                                                                 \
+            hipLaunchKernelGGL((k_sweep<K_, T_, W_, L2O_SWEEP_PRUNE>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
+        else if (mode == L2O_SWEEP_PRUNE_T1)                                                         \
+            hipLaunchKernelGGL((k_sweep<K_, T_, W_, L2O_SWEEP_PRUNE_T1>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
         else                                                                                         \
-            hipLaunchKernelGGL((k_sweep<K_, T_, W_, false>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
+            hipLaunchKernelGGL((k_sweep<K_, T_, W_, L2O_SWEEP_FULL>), dim3(nitems), dim3(64 * W_), 0, stream, items, insts); \
         return hipGetLastError();                                                                    \
     }
 
 hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint32_t nitems,
-                        l2o_inst *insts, hipStream_t stream, bool prune)
+                        l2o_inst *insts, hipStream_t stream, int mode)
 {
     if (nitems == 0) return hipSuccess;
+    if (mode < L2O_SWEEP_FULL || mode > L2O_SWEEP_PRUNE_T1) return hipErrorInvalidValue;
     L2O_SWEEP_SHAPES(L2O_SWEEP_CASE)
     return hipErrorInvalidValue;
 }

@@ -77,10 +77,14 @@ struct sweep_cfg {
 bool sweep_cfg_supported(const sweep_cfg &c);
 // compiled shapes: fills up to cap rows {k, t, waves}, returns the total count
 int sweep_shapes(int (*out)[3], int cap);
-// one workgroup per item; prune selects the variant that skips blocks of
-// rows a bounding-box bound rules out (same results)
+// Sweep variants (same results): the full sweep (every block of rows through
+// tier 0), and row pruning, which first skips the blocks of rows a
+// bounding-box bound rules out and sends the rest through tier 0
+// (L2O_SWEEP_PRUNE) or straight to tier 1 (L2O_SWEEP_PRUNE_T1)
+enum { L2O_SWEEP_FULL = 0, L2O_SWEEP_PRUNE = 1, L2O_SWEEP_PRUNE_T1 = 2 };
+// one workgroup per item, of the variant `mode`
 hipError_t launch_sweep(const sweep_cfg &cfg, const l2o_sweep_item *items, uint32_t nitems,
-                        l2o_inst *insts, hipStream_t stream, bool prune = false);
+                        l2o_inst *insts, hipStream_t stream, int mode = L2O_SWEEP_FULL);
 hipError_t launch_apply(const l2o_apply_item *items, uint32_t nitems, l2o_inst *insts, unsigned *live,
                         hipStream_t stream);
 hipError_t launch_ils(l2o_inst *insts, uint32_t nslots, unsigned *live, hipStream_t stream);

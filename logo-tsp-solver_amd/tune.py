@@ -12,6 +12,9 @@ use from its own directory:
                            an engine holding enough instances of that size to
                            fill the GPU (logo2opt_engine_time_sweep), and the
                            time per candidate move decides
+One table serves both sweep variants, so a shape is scored on the full sweep
+plus, where row pruning is in use (batches, and single instances from the
+size LOGO2OPT_PRUNE_AUTO prunes), the pruned sweep: the sum of the two times.
 A shape is {k columns per lane, t rows, waves per workgroup}.  Bucket edges
 are the geometric midpoints between the representative sizes.
 """
@@ -28,6 +31,7 @@ import logo2opt as L  # noqa: E402
 
 REPS = (100, 250, 500, 1000, 2000, 4000, 8000, 16000, 33810, 100000)
 BATCH_REPS = (100, 250, 500, 1000, 2000, 4000, 8000, 16000)
+PRUNE_AUTO_MIN_N = 16000  # LOGO2OPT_PRUNE_AUTO prunes from this size (l2o_engine.hip)
 
 
 def pairs(n):
@@ -52,18 +56,21 @@ def tune_single(reps=REPS, seed=1, verbose=True):
             xy = L.gen_uniform(n, side, seed + n) if kind == "u" else L.gen_clustered(n, side, seed + n)
             cases.append((L.coords_of(xy), L.nn_tour(xy, 0)))
         steps = 0xFFFFFFFF if n <= 40000 else 400
+        modes = (L.PRUNE_OFF, L.PRUNE_ALL) if n >= PRUNE_AUTO_MIN_N else (L.PRUNE_OFF,)
         times = {}
         for sh in L.tuning_shapes():
             L.tuning_set([(L.MAX_N, sh)], L.TABLE_SINGLE)
             tot = 0.0
             for coords, r0 in cases:
-                with L.Engine(hbm_budget=L.Engine.bytes_for(n) + (16 << 20), max_slots=1) as eng:
-                    eng.add(coords, r0)
-                    if eng.run(max_steps=64, sync_every=64) == 0:
-                        continue  # converged during the warm-up: nothing to time
-                    t0 = time.perf_counter()
-                    eng.run(max_steps=steps, sync_every=64)
-                    tot += time.perf_counter() - t0
+                for mode in modes:
+                    with L.Engine(hbm_budget=L.Engine.bytes_for(n) + (16 << 20), max_slots=1) as eng:
+                        eng.set_pruning(mode)
+                        eng.add(coords, r0)
+                        if eng.run(max_steps=64, sync_every=64) == 0:
+                            continue  # converged during the warm-up: nothing to time
+                        t0 = time.perf_counter()
+                        eng.run(max_steps=steps, sync_every=64)
+                        tot += time.perf_counter() - t0
             times[sh] = tot
         b = min(times, key=times.get)
         best.append((n, b))
@@ -84,8 +91,11 @@ def tune_batch(reps=BATCH_REPS, seed=2, verbose=True):
         times = {}
         for sh in L.tuning_shapes():
             L.tuning_set([(L.MAX_N, sh)], L.TABLE_BATCH)
-            eng.time_sweep(1)
-            times[sh] = eng.time_sweep(5) / (m * pairs(n))
+            times[sh] = 0.0
+            for mode in (L.PRUNE_OFF, L.PRUNE_ALL):
+                eng.set_pruning(mode)
+                eng.time_sweep(1)
+                times[sh] += eng.time_sweep(5) / (m * pairs(n))
         eng.close()
         b = min(times, key=times.get)
         best.append((n, b))
