@@ -251,10 +251,13 @@ __device__ __forceinline__ int wave_max_i32(int v)
 // ceil(W[b+1] / Q) and wa = ceil(W[a+1] / Q), a tier-1 hit has
 //   Sq - wq - wa <= floor((tb + 4 (Q - 1)) / Q),
 // so a block of steps in which no candidate meets this holds no tier-1 hit
-// and is skipped.  Each wave stages its rows' packed byte pairs and wa in LDS
-// once (one lane per row), and reads them back as broadcasts, so a step is
-// VALU work only: one subtract, six v_sad_u8 and three v_min3 for six
-// candidates.  Sq <= 1,020 and wq, wa < 2^10; the sum carries a bias of 2^16
+// and is skipped.  A block is tested with its largest wa for each of its
+// rows (a looser, valid test: one subtract per block, then per step K
+// v_sad_u8 and K/2 v_min3; with each row's own wa the test costs a subtract
+// per step, and re-testing the blocks that pass with them measured slower).
+// Each wave stages its rows' packed byte pairs and the blocks' wa in LDS once
+// (one lane per block) and reads them back as broadcasts, so the test is VALU
+// work only.  Sq <= 1,020 and wq, wa < 2^10; the sum carries a bias of 2^16
 // so that it stays positive.
 #ifndef L2O_TIER0
 #define L2O_TIER0 1
@@ -386,29 +389,46 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         wmax = wave_max_i32(wmax);
     }
     // TIER0: column pairs (b, b+1) of the lane as bytes of coordinates >> qs,
-    // and the wave's rows in LDS: t0rp[t] = the same bytes of rows (a, a+1),
-    // t0wa[t] = ceil(W[a+1] / 2^qs), a = a0+t, for the rows of whole blocks
-    // (a+1 <= a0 + t_end <= n-3).  Only this wave reads its rows, and LDS
+    // and the wave's blocks of rows in LDS, 32 bytes each: for block q (rows
+    // 4q .. 4q+3) the same bytes of rows (a, a+1), a = a0+4q+s, s = 0..3, then
+    // the block's largest wa = ceil(W[a+1] / 2^qs), for the whole blocks (a+1
+    // <= a0 + t_end <= n-3).  Only this wave reads its blocks, and LDS
     // operations of one wave complete in order: no barrier.
+    constexpr int T0W = 2 * T;  // LDS words per wave (8 per block)
     uint32_t cpk[TIER0 ? K : 1];
     uint32_t c0 = 0;
-    __shared__ __attribute__((aligned(16))) uint32_t t0lds[TIER0 ? 2 * WAVES * T : 1];
-    uint32_t *const t0rp = t0lds + (TIER0 ? 2 * w * T : 0);
-    uint32_t *const t0wa = t0rp + (TIER0 ? T : 0);
+    __shared__ __attribute__((aligned(16))) uint32_t t0lds[TIER0 ? WAVES * T0W : 1];
+    uint4 *const t0blk = (uint4 *)(t0lds + (TIER0 ? w * T0W : 0));  // 2 per block
     auto t0_pack = [&](uint32_t xa, uint32_t ya, uint32_t xb, uint32_t yb) __attribute__((always_inline)) {
         return (xa >> qs) | ((ya >> qs) << 8) | ((xb >> qs) << 16) | ((yb >> qs) << 24);
     };
-    auto t0_wq = [&](uint32_t wb) __attribute__((always_inline)) { return (wb + (1u << qs) - 1) >> qs; };
+    // block q from its five points (rows 4q .. 4q+4)
+    auto t0_stage = [&](int q, const int *x, const int *y, const int *wr) __attribute__((always_inline)) {
+        uint4 v;
+        v.x = t0_pack(x[0], y[0], x[1], y[1]);
+        v.y = t0_pack(x[1], y[1], x[2], y[2]);
+        v.z = t0_pack(x[2], y[2], x[3], y[3]);
+        v.w = t0_pack(x[3], y[3], x[4], y[4]);
+        t0blk[2 * q] = v;
+        const uint32_t mw = (uint32_t)max(max(wr[1], wr[2]), max(wr[3], wr[4]));
+        t0blk[2 * q + 1].x = (mw + (1u << qs) - 1) >> qs;
+    };
     if constexpr (TIER0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) cpk[k] = t0_pack(cx[k], cy[k], cx[k + 1], cy[k + 1]);
         c0 = L2O_T0_BIAS - (uint32_t)wq;
         if constexpr (!PRUNE) {
-            // one lane per row (PRUNE: from the row test's points, below)
-            const int rows0 = max(t_end, 0) / UF * UF;
-            for (int t = lane; t < rows0; t += 64) {
-                t0rp[t] = t0_pack(P[a0 + t].x, P[a0 + t].y, P[a0 + t + 1].x, P[a0 + t + 1].y);
-                t0wa[t] = t0_wq(P[a0 + t + 1].w);
+            // one lane per block (PRUNE: from the row test's points, below)
+            const int nb = max(t_end, 0) / UF;
+            for (int q = lane; q < nb; q += 64) {
+                int x[UF + 1], y[UF + 1], wr[UF + 1];
+#pragma unroll
+                for (int s = 0; s <= UF; ++s) {
+                    x[s] = P[a0 + UF * q + s].x;
+                    y[s] = P[a0 + UF * q + s].y;
+                    wr[s] = P[a0 + UF * q + s].w;
+                }
+                t0_stage(q, x, y, wr);
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -476,13 +496,7 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         if constexpr (TIER0) {
             // lane l holds the points of block l (rows 4l .. 4l+4), which are
             // unclamped for a whole block
-            if (keep && r0 + UF <= t_end) {
-#pragma unroll
-                for (int s = 0; s < UF; ++s) {
-                    t0rp[r0 + s] = t0_pack(qx[s], qy[s], qx[s + 1], qy[s + 1]);
-                    t0wa[r0 + s] = t0_wq(qw[s + 1]);
-                }
-            }
+            if (keep && r0 + UF <= t_end) t0_stage(lane, qx, qy, qw);
             __builtin_amdgcn_wave_barrier();
         }
         const unsigned long long g = key_now();
@@ -623,13 +637,12 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         using UF1c = std::integral_constant<int, UF + 1>;
         const int nmain = max(t_end, 0) / UF;  // whole blocks
         int tv = 0;                    // V holds row a0+tv
-        const uint4 *rp4 = (const uint4 *)t0rp, *wa4 = (const uint4 *)t0wa;
-        auto process = [&](int q, const uint4 rp, const uint4 wa) __attribute__((always_inline)) {
+        auto process = [&](int q, const uint4 rp, const uint32_t wb) __attribute__((always_inline)) {
             int m = 0x7FFFFFFF;
-            const uint32_t rs[UF] = {rp.x, rp.y, rp.z, rp.w}, ws[UF] = {wa.x, wa.y, wa.z, wa.w};
+            const uint32_t rs[UF] = {rp.x, rp.y, rp.z, rp.w};
+            const uint32_t c = c0 - wb;
 #pragma unroll
             for (int s = 0; s < UF; ++s) {
-                const uint32_t c = c0 - ws[s];
 #pragma unroll
                 for (int k = 0; k < K; ++k) m = min(m, (int)__builtin_amdgcn_sad_u8(cpk[k], rs[s], c));
             }
@@ -653,10 +666,10 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
             while (todo) {
                 const int q = __builtin_ctzll(todo);
                 todo &= todo - 1;
-                process(q, rp4[q], wa4[q]);
+                process(q, t0blk[2 * q], t0blk[2 * q + 1].x);
             }
         } else {
-            for (int q = 0; q < nmain; ++q) process(q, rp4[q], wa4[q]);
+            for (int q = 0; q < nmain; ++q) process(q, t0blk[2 * q], t0blk[2 * q + 1].x);
         }
         // the rows after the last whole block (PRUNE: if the bound left any):
         // tier 1 directly

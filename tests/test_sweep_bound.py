@@ -179,10 +179,18 @@ def _tier0_cands(X, Y, q):
 def _check_tier0(X, Y, lane_cols=6):
     """Every candidate that passes tier 1 (L - W[b+1] <= tb + W[a+1]) at any
     threshold passes tier 0, with wq = ceil(W[b+1]/2^q) and with the lane's
-    largest such value (columns grouped lane_cols to a lane)."""
+    largest such value (columns grouped lane_cols to a lane), and with W[a+1]
+    replaced by the largest of its block of four rows (a // 4), as the kernel
+    stages it."""
     q = tier0_shift(int(max(np.max(X), np.max(Y))))
     assert max(np.max(X), np.max(Y)) >> q <= 255
     L, wa, wb, Sq, b = _tier0_cands(X, Y, q)
+    n = len(X)
+    a = np.array([a for a in range(n - 3) for _ in range(a + 2, n - 1)])
+    blk = np.zeros(a.max() // 4 + 1, np.int64)
+    np.maximum.at(blk, a // 4, wa)
+    wa_blk = blk[a // 4]
+    assert np.all(wa_blk >= wa)
     Q = 1 << q
     wq = -(-wb // Q)
     lane = (b - 2) // lane_cols
@@ -195,6 +203,7 @@ def _check_tier0(X, Y, lane_cols=6):
         t1 = slack <= tb
         assert np.all(_tier0_keeps(Sq, wq, tb, wa, q)[t1]), (q, tb)
         assert np.all(_tier0_keeps(Sq, wq_lane[lane], tb, wa, q)[t1]), (q, tb)
+        assert np.all(_tier0_keeps(Sq, wq_lane[lane], tb, wa_blk, q)[t1]), (q, tb)
         if q == 0:  # no shift: tier 0 is tier 1
             assert np.array_equal(_tier0_keeps(Sq, wq, tb, wa, q), t1)
     return q
