@@ -1151,7 +1151,7 @@ namespace l2o {
     X(1, 32, 1) X(1, 64, 1) X(2, 64, 1) X(2, 128, 1) X(4, 64, 1) X(4, 128, 1) X(4, 256, 1)           \
     X(6, 64, 1) X(6, 128, 1) X(8, 64, 1) X(8, 128, 1) X(8, 256, 1) X(4, 64, 2) X(4, 128, 2)           \
     X(6, 64, 2) X(8, 64, 2) X(8, 128, 2) X(8, 256, 2) X(2, 64, 4) X(4, 64, 4) X(6, 64, 4)             \
-    X(8, 64, 4) X(8, 128, 4) X(6, 128, 2) X(6, 160, 1) X(6, 128, 4)
+    X(8, 64, 4) X(8, 128, 4) X(6, 128, 2) X(6, 160, 1) X(6, 128, 4) X(4, 256, 2) X(4, 256, 4)
 
 #define L2O_SWEEP_CASE(K_, T_, W_)                                                                   \
     if (cfg.k == K_ && cfg.t == T_ && cfg.waves == W_) {                                            \

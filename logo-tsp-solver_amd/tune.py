@@ -9,9 +9,8 @@ use from its own directory:
                            shape is timed on the steps of a local search from
                            the nearest-neighbour start tour
   tuning_batch_gfx950.txt  batches (8+ live instances): each shape is timed on
-                           an engine holding enough instances of that size to
-                           fill the GPU (logo2opt_engine_time_sweep), and the
-                           time per candidate move decides
+                           an engine holding enough random-start instances of
+                           that size to fill the GPU, run to their local optima
 One table serves both sweep variants, so a shape is scored on the full sweep
 plus, where row pruning is in use (batches, and single instances from the
 size LOGO2OPT_PRUNE_AUTO prunes), the pruned sweep: the sum of the two times.
@@ -32,6 +31,7 @@ import logo2opt as L  # noqa: E402
 REPS = (100, 250, 500, 1000, 2000, 4000, 8000, 16000, 33810, 100000)
 BATCH_REPS = (100, 250, 500, 1000, 2000, 4000, 8000, 16000)
 PRUNE_AUTO_MIN_N = 16000  # LOGO2OPT_PRUNE_AUTO prunes from this size (l2o_engine.hip)
+BATCH_STEPS = 3000  # batched steps timed per shape from 8,000 cities
 
 
 def pairs(n):
@@ -81,28 +81,41 @@ def tune_single(reps=REPS, seed=1, verbose=True):
 
 
 def tune_batch(reps=BATCH_REPS, seed=2, verbose=True):
+    """Times every shape on whole batched local searches: an engine holding
+    enough random-start instances of the size to fill the GPU (about 4e8
+    candidates per batched sweep) runs them to their local optima (the first
+    8 steps untimed; from 8,000 cities the next BATCH_STEPS steps only), once
+    with the full sweep and once pruned, and the sum of the two times decides.
+    Timing single sweeps of the random start tours instead (the earlier
+    objective) favoured shapes that were slower over whole searches, where
+    the tours improve and the filters reject more."""
     best = []
     for n in reps:
-        m = max(8, min(1024, int(4e8 / pairs(n))))  # ~4e8 candidates per batched sweep
-        eng = L.Engine(hbm_budget=m * L.Engine.bytes_for(n) + (16 << 20), max_slots=m)
+        m = max(8, min(1024, int(4e8 / pairs(n))))
+        insts = []
         for q in range(m):
             xy = L.gen_uniform(n, 10**4, seed * 100000 + q)
-            eng.add(L.coords_of(xy), L.random_perm(n, seed + q))
+            insts.append((L.coords_of(xy), L.random_perm(n, seed + q)))
+        steps = 0xFFFFFFFF if n < 8000 else BATCH_STEPS
         times = {}
         for sh in L.tuning_shapes():
             L.tuning_set([(L.MAX_N, sh)], L.TABLE_BATCH)
-            times[sh] = 0.0
+            tot = 0.0
             for mode in (L.PRUNE_OFF, L.PRUNE_ALL):
-                eng.set_pruning(mode)
-                eng.time_sweep(1)
-                times[sh] += eng.time_sweep(5) / (m * pairs(n))
-        eng.close()
+                with L.Engine(hbm_budget=m * L.Engine.bytes_for(n) + (16 << 20), max_slots=m) as eng:
+                    eng.set_pruning(mode)
+                    for coords, r0 in insts:
+                        eng.add(coords, r0)
+                    eng.run(max_steps=8, sync_every=8)
+                    t0 = time.perf_counter()
+                    eng.run(max_steps=steps, sync_every=32)
+                    tot += time.perf_counter() - t0
+            times[sh] = tot
         b = min(times, key=times.get)
         best.append((n, b))
         if verbose:
             ranked = sorted(times.items(), key=lambda kv: kv[1])[:4]
-            print(f"batch  n={n:>6} (x{m}): " + "  ".join(f"{s}:{ms * 1e-3 * 1e12:.3f}ps/eval" for s, ms in ranked),
-                  flush=True)
+            print(f"batch  n={n:>6} (x{m}): " + "  ".join(f"{s}:{t * 1e3:.1f}ms" for s, t in ranked), flush=True)
     return best
 
 
