@@ -309,11 +309,13 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     const bool diag = bw < a0 + t_end + 1;
     if (diag) t_end = min(t_end, min(bw + WC - 1, n - 2) - 1 - a0);
 
-    // PRUNE: the points of the row test below (lane l: rows a0+4l .. a0+4l+4),
+    // PRUNE, TIER0: the points of block l of the rows for lane l (rows
+    // a0+4l .. a0+4l+4), for the row test and the tier-0 staging below,
     // requested before the columns (their addresses do not depend on them);
     // indices are clamped to a0 + t_end <= n - 3
-    int qx[PRUNE ? UF + 1 : 1], qy[PRUNE ? UF + 1 : 1], qw[PRUNE ? UF + 1 : 1];
-    if constexpr (PRUNE) {
+    constexpr bool QROWS = PRUNE || TIER0;
+    int qx[QROWS ? UF + 1 : 1], qy[QROWS ? UF + 1 : 1], qw[QROWS ? UF + 1 : 1];
+    if constexpr (QROWS) {
 #pragma unroll
         for (int s = 0; s <= UF; ++s) {
             const uint32_t q = (uint32_t)min(a0 + lane * UF + s, a0 + max(t_end, 0));
@@ -418,18 +420,8 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         for (int k = 0; k < K; ++k) cpk[k] = t0_pack(cx[k], cy[k], cx[k + 1], cy[k + 1]);
         c0 = L2O_T0_BIAS - (uint32_t)wq;
         if constexpr (!PRUNE) {
-            // one lane per block (PRUNE: from the row test's points, below)
-            const int nb = max(t_end, 0) / UF;
-            for (int q = lane; q < nb; q += 64) {
-                int x[UF + 1], y[UF + 1], wr[UF + 1];
-#pragma unroll
-                for (int s = 0; s <= UF; ++s) {
-                    x[s] = P[a0 + UF * q + s].x;
-                    y[s] = P[a0 + UF * q + s].y;
-                    wr[s] = P[a0 + UF * q + s].w;
-                }
-                t0_stage(q, x, y, wr);
-            }
+            // one lane per whole block (PRUNE: after the row test, below)
+            if (lane < max(t_end, 0) / UF) t0_stage(lane, qx, qy, qw);
             __builtin_amdgcn_wave_barrier();
         }
     }
