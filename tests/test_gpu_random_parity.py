@@ -1,8 +1,8 @@
 """Seeded random instances through the C ABI against the oracle, bit for bit:
 sizes log-uniform in [4, 1500], every metric, uniform or clustered points,
-coordinate spans from 10 to 2^22 - 1, random or nearest-neighbour starts,
-row pruning on or off; the first sweep's move, the whole local search's move
-log, length and tour.  LOGO2OPT_RANDOM_PARITY_COUNT sets the number of
+coordinate spans from 10 to 2^22 - 1 (tier-0 shifts 0 to 14), random or
+nearest-neighbour starts, row pruning on or off; the first sweep's move, the
+whole local search's move log, length and tour.  LOGO2OPT_RANDOM_PARITY_COUNT sets the number of
 instances (default 24)."""
 import os
 import sys
@@ -27,7 +27,7 @@ def cases(count, seed=2026):
     rs = np.random.default_rng(seed)
     for q in range(count):
         n = int(np.rint(np.exp(rs.uniform(np.log(4), np.log(1500)))))
-        side = int(rs.choice([10, 100, 10**4, 10**6, (1 << 22) - 1]))
+        side = int(rs.choice([10, 100, 1000, 10**4, 1 << 16, 10**6, (1 << 22) - 1]))
         metric = int(rs.integers(0, 3))
         clustered = bool(rs.random() < 0.5) and side >= 10**4
         nn = bool(rs.random() < 0.5)
