@@ -127,6 +127,29 @@ int logo2opt_optimize(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t *length
 int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t rounds,
                  int kick, uint64_t *best_length, uint64_t *round_lengths);
 
+/* ILSGlobalSolverSequential::optimize with the reference's stop rules
+ * (ILSGlobalSolverSequential.cpp:60-100; solution, error and timelimit of
+ * ILSGlobalSolver.h:160-190).  After the first LO and after every round the
+ * search ends once
+ *   - the best length L satisfies !(solution * (1.0f + err) < (float)L),
+ *     evaluated in float as the reference's loop condition is;
+ *   - time_limit_s > 0 seconds have passed since the call began (checked by
+ *     the host every 64 device steps; the round in progress then completes,
+ *     as the reference checks its clock after each round); or
+ *   - max_rounds rounds have run.
+ * route_inout receives the best tour, round_lengths (optional, max_rounds+1
+ * entries) the best length after the first LO and each round run,
+ * rounds_done (optional) the rounds run. */
+int logo2opt_ils_until(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t max_rounds,
+                       int kick, uint64_t solution, float err, double time_limit_s,
+                       uint64_t *best_length, uint64_t *round_lengths, uint32_t *rounds_done);
+
+/* The job's stop_below for the reference's target rule: one more than the
+ * largest L with !(solution * (1.0f + err) < (float)L) (no counterpart in
+ * the reference: the conversion of its loop condition to an integer bound;
+ * 0 for an unreachable target). */
+uint64_t logo2opt_stop_below(uint64_t solution, float err);
+
 /* ILSGlobalSolverMT::optimize with comm = 1 and `chains` threads
  * (ILSGlobalSolverMT.cpp:86-195): chains of ILS on the same instance that
  * share a global best, here race free.  All chains start from route_inout and
@@ -201,6 +224,8 @@ typedef struct {
     uint64_t seed;         /* kick generator seed                                 */
     uint32_t log_cap;      /* moves to log (0 = none)                             */
     uint32_t metric;       /* LOGO2OPT_METRIC_* (0 = EUC_2D)                      */
+    uint64_t stop_below;   /* ILS: stop after the first local search or round     */
+                           /* whose best length is below this (0 = never)         */
 } logo2opt_job;
 
 typedef struct {

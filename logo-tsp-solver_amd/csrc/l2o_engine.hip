@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -380,6 +381,7 @@ int upload_instance(logo2opt_engine *e, int32_t id, const uint32_t *route, uint3
         for (const int32_t v : s.xy) m = std::max(m, v);
         in.qshift = l2o::tier0_shift((uint32_t)m);
     }
+    in.stop_below = s.job.ils_rounds ? s.job.stop_below : 0;
     in.phase = phase;
     in.key = L2O_SENTINEL;
     // SplitMix64 -> xoshiro256** state (same seeding as the workload generators)
@@ -1316,6 +1318,72 @@ int logo2opt_ils(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32
     if (best_length) *best_length = res.length;
     if (round_lengths) {
         rc = logo2opt_engine_round_lengths(e, ctx->slot, round_lengths, rounds + 1);
+        if (rc) return rc;
+    }
+    return LOGO2OPT_OK;
+}
+
+uint64_t logo2opt_stop_below(uint64_t solution, float err)
+{
+    // the reference's loop goes on while solution * (1.0f + error) < best,
+    // in float (ILSGlobalSolverSequential.cpp:98); the largest best that ends
+    // it, found by bisection (the float conversion is monotone)
+    const float thr = (float)solution * (1.0f + err);
+    auto ends = [&](uint64_t L) { return !(thr < (float)L); };
+    if (!ends(0)) return 0;  // unreachable target (thr is negative or NaN)
+    if (ends(UINT64_MAX)) return UINT64_MAX;  // every length ends it
+    uint64_t lo = 0, hi = UINT64_MAX;  // ends(lo), !ends(hi)
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (ends(mid)) lo = mid;
+        else hi = mid;
+    }
+    return lo + 1;
+}
+
+int logo2opt_ils_until(logo2opt_ctx *ctx, uint32_t *route_inout, uint64_t seed, uint32_t max_rounds, int kick,
+                       uint64_t solution, float err, double time_limit_s, uint64_t *best_length,
+                       uint64_t *round_lengths, uint32_t *rounds_done)
+{
+    if (!ctx || !route_inout || (kick != 0 && kick != 1) || max_rounds == 0 || !(time_limit_s >= 0.0))
+        return LOGO2OPT_EINVAL;
+    logo2opt_engine *e = ctx->e;
+    if (hipSetDevice(e->device) != hipSuccess) return LOGO2OPT_EDEVICE;
+    const double t_start = now_ms();
+    logo2opt_job job{};
+    job.ils_rounds = max_rounds;
+    job.kick = (uint32_t)kick;
+    job.seed = seed;
+    job.stop_below = logo2opt_stop_below(solution, err);
+    int rc = ctx_load(ctx, route_inout, job);
+    if (rc) return rc;
+    e = ctx->e;  // ctx_load replaces the engine when it needs a larger one
+    const double t0 = now_ms();
+    uint32_t live = 1;
+    bool capped = false;
+    while (live) {
+        rc = engine_run(e, 64, 64, &live);
+        if (rc) return rc;
+        if (live && !capped && time_limit_s > 0.0 && (now_ms() - t_start) * 1e-3 >= time_limit_s) {
+            // the clock has run out: the round in progress is the last one
+            l2o_inst in;
+            L2O_TRY(hipMemcpyAsync(&in, e->d_insts + ctx->slot, sizeof(in), hipMemcpyDeviceToHost, e->stream));
+            L2O_TRY(hipStreamSynchronize(e->stream));
+            const uint32_t cap = std::min(in.ils_rounds, in.rounds_done);
+            L2O_TRY(hipMemcpyAsync((char *)(e->d_insts + ctx->slot) + offsetof(l2o_inst, ils_rounds), &cap,
+                                   sizeof(cap), hipMemcpyHostToDevice, e->stream));
+            L2O_TRY(hipStreamSynchronize(e->stream));
+            capped = true;
+        }
+    }
+    ctx->wall_ms += now_ms() - t0;
+    logo2opt_result res;
+    rc = engine_result(e, ctx->slot, route_inout, &res, nullptr);
+    if (rc) return rc;
+    if (best_length) *best_length = res.length;
+    if (rounds_done) *rounds_done = res.rounds_done;
+    if (round_lengths) {
+        rc = logo2opt_engine_round_lengths(e, ctx->slot, round_lengths, max_rounds + 1);
         if (rc) return rc;
     }
     return LOGO2OPT_OK;

@@ -898,7 +898,8 @@ __device__ unsigned long long xoshiro_next(unsigned long long *s)
 }
 
 // A plain ILS instance at L2O_LO_DONE: accept iff strictly shorter (copy to
-// the best buffers), stop after `rounds`, else kick its best and restart.
+// the best buffers), stop after `rounds` or once the best length is below
+// stop_below, else kick its best and restart.
 // A chain of a multi-start group at L2O_LO_DONE only accepts into its own best
 // and waits (L2O_WAIT) for k_group; at L2O_KICK it kicks the best tour of
 // `kick_src` that k_group chose.  While any chain of a group is at L2O_KICK,
@@ -926,7 +927,7 @@ __global__ __launch_bounds__(L2O_ILS_THREADS) void k_ils(l2o_inst *__restrict__ 
                 in->improved = acc;
             } else {
                 if (in->rlog) in->rlog[in->rounds_done] = in->best_length;
-                done = in->rounds_done >= in->ils_rounds;
+                done = in->rounds_done >= in->ils_rounds || in->best_length < in->stop_below;
             }
         }
         s_accept = acc;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(64) void k_group(l2o_inst *__restrict__ insts, uint
     }
     const unsigned long long G = insts[best].best_length;
     const uint32_t rounds = lead->rounds_done;
-    const bool done = rounds >= lead->ils_rounds;
+    const bool done = rounds >= lead->ils_rounds || G < lead->stop_below;
     for (uint32_t c = 0; c < m; ++c) {
         l2o_inst *in = insts + id + c;
         if (in->rlog) in->rlog[rounds] = G;

@@ -70,6 +70,7 @@ struct l2o_inst {
     float est_scale;      // FP32 tier: D ~ sqrt(s) * est_scale + est_add
     float est_add;
     uint32_t qshift;      // tier 0: coordinates >> qshift fit in 8 bits
+    unsigned long long stop_below;  // ILS: done once best_length < stop_below (0: never)
     // ---- dynamic
     uint32_t phase;
     uint32_t parity;

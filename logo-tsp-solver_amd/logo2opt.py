@@ -43,7 +43,7 @@ EXPORTS = [
     "logo2opt_tour_write", "logo2opt_host_route_length", "logo2opt_mf_tour", "logo2opt_ils_multistart",
     "logo2opt_step_noalloc",
     "logo2opt_engine_add_group", "logo2opt_engine_group_result", "logo2opt_set_pruning",
-    "logo2opt_engine_set_pruning",
+    "logo2opt_engine_set_pruning", "logo2opt_ils_until", "logo2opt_stop_below",
 ]
 
 
@@ -68,7 +68,7 @@ class Stats(C.Structure):
 
 class Job(C.Structure):
     _fields_ = [("ils_rounds", C.c_uint32), ("kick", C.c_uint32), ("seed", C.c_uint64),
-                ("log_cap", C.c_uint32), ("metric", C.c_uint32)]
+                ("log_cap", C.c_uint32), ("metric", C.c_uint32), ("stop_below", C.c_uint64)]
 
 
 class TspInfo(C.Structure):
@@ -118,6 +118,10 @@ def load():
     L.logo2opt_route_length.argtypes = [vp, u32p, C.POINTER(C.c_uint64)]
     L.logo2opt_ils_multistart.argtypes = [vp, u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int,
                                           C.POINTER(C.c_uint64), vp]
+    L.logo2opt_ils_until.argtypes = [vp, u32p, C.c_uint64, C.c_uint32, C.c_int, C.c_uint64, C.c_float,
+                                     C.c_double, C.POINTER(C.c_uint64), vp, C.POINTER(C.c_uint32)]
+    L.logo2opt_stop_below.argtypes = [C.c_uint64, C.c_float]
+    L.logo2opt_stop_below.restype = C.c_uint64
     L.logo2opt_engine_add_group.argtypes = [vp, C.c_uint32, vp, u32p, C.POINTER(Job), C.c_uint32,
                                             C.POINTER(C.c_int32)]
     L.logo2opt_engine_group_result.argtypes = [vp, C.c_int32, vp, C.POINTER(Result), C.POINTER(C.c_int32)]
@@ -266,6 +270,11 @@ def host_route_length(coords, route, metric=METRIC_EUC_2D):
 
 
 # ---- sweep tile shapes -------------------------------------------------------
+def stop_below(solution, err):
+    """The job field for the reference's ILS target rule (logo2opt_stop_below)."""
+    return int(load().logo2opt_stop_below(solution, err))
+
+
 def tuning_shapes():
     out = np.zeros(3 * 64, np.int32)
     cnt = C.c_uint32()
@@ -358,6 +367,17 @@ class Context:
                "logo2opt_ils")
         return r, int(best.value), rl
 
+    def ils_until(self, route, seed, max_rounds, solution, err=0.0, time_limit_s=0.0, kick=KICK_DOUBLE_BRIDGE):
+        """ILS with the reference's stop rules: returns (best tour, best length,
+        round lengths of the rounds run, rounds run)."""
+        r = np.array(route, np.uint32)
+        best = C.c_uint64()
+        done = C.c_uint32()
+        rl = np.zeros(max_rounds + 1, np.uint64)
+        _check(load().logo2opt_ils_until(self.h, r, seed, max_rounds, kick, solution, err, time_limit_s,
+                                         C.byref(best), rl.ctypes.data, C.byref(done)), "logo2opt_ils_until")
+        return r, int(best.value), rl[:done.value + 1], int(done.value)
+
     def ils_multistart(self, route, seed, rounds, chains, kick=KICK_DOUBLE_BRIDGE):
         r = np.array(route, np.uint32)
         best = C.c_uint64()
@@ -422,10 +442,11 @@ class Engine:
         j = Job(ils_rounds, 0, 0, log_cap, 0)
         return int(load().logo2opt_engine_bytes_for(n, C.byref(j)))
 
-    def add(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, log_cap=0, metric=METRIC_EUC_2D):
+    def add(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, log_cap=0, metric=METRIC_EUC_2D,
+            stop_below=0):
         """Returns the slot id, or None when the engine is full (EAGAIN)."""
         route = np.ascontiguousarray(route, np.uint32)
-        j = Job(ils_rounds, kick, seed, log_cap, metric)
+        j = Job(ils_rounds, kick, seed, log_cap, metric, stop_below)
         s = C.c_int32()
         rc = load().logo2opt_engine_add(self.h, len(route), C.cast(coords, vp), route, C.byref(j), C.byref(s))
         if rc == -6:
@@ -433,10 +454,11 @@ class Engine:
         _check(rc, "logo2opt_engine_add")
         return s.value
 
-    def add_group(self, coords, route, chains, ils_rounds, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D):
+    def add_group(self, coords, route, chains, ils_rounds, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D,
+                  stop_below=0):
         """Multi-start ILS group; returns its first slot, or None when full."""
         route = np.ascontiguousarray(route, np.uint32)
-        j = Job(ils_rounds, kick, seed, 0, metric)
+        j = Job(ils_rounds, kick, seed, 0, metric, stop_below)
         s = C.c_int32()
         rc = load().logo2opt_engine_add_group(self.h, len(route), C.cast(coords, vp), route, C.byref(j), chains,
                                               C.byref(s))

@@ -41,7 +41,7 @@ def test_struct_layouts_match_reference():
     assert C.sizeof(L.CityCoords) == 8
     assert C.sizeof(L.Best2Out) == 12
     assert C.sizeof(L.ProcessTime) == 3 * 8 + 12 + 4  # padded to 8
-    assert C.sizeof(L.Job) == 24 and C.sizeof(L.Result) == 40
+    assert C.sizeof(L.Job) == 32 and C.sizeof(L.Result) == 40
 
 
 def test_strerror_codes():
