@@ -579,9 +579,11 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D, n=None):
-        """route None: the server builds the greedy NN tour from city 0 (give n)."""
-        j = Job(ils_rounds, kick, seed, 0, metric)
+    def submit(self, coords, route, ils_rounds=0, kick=KICK_DOUBLE_BRIDGE, seed=0, metric=METRIC_EUC_2D, n=None,
+               stop_below=0):
+        """route None: the server builds the greedy NN tour from city 0 (give n).
+        stop_below: the ILS target bound (see stop_below())."""
+        j = Job(ils_rounds, kick, seed, 0, metric, stop_below)
         rid = C.c_int64()
         if route is None:
             rc = load().logo2opt_server_submit(self.h, n, C.cast(coords, vp), None, C.byref(j), C.byref(rid))
