@@ -49,9 +49,9 @@ METRIC = "2-opt move evals/sec + instances/sec batched vs plain, bit-exact tour 
 # does less (integer L1 bounds, see l2o_kernels.hip) and is priced the same
 FLOP_PER_EVAL = 14
 # HBM bytes per sweep launch of the 33,810-city search, from two rocprofv3 --pmc runs of this
-# benchmark (FETCH_SIZE 2,680.8 KiB x 2.0 + WRITE_SIZE 4.56 KiB x 1.0, factors from a 256 MiB copy;
+# benchmark (FETCH_SIZE 2,680.2 KiB x 2.0 + WRITE_SIZE 4.42 KiB x 1.0, factors from a 256 MiB copy;
 # profiles/k_sweep_config2_counters.md).  Reported only for the kernel that was profiled.
-PROFILED_TRAFFIC = {"l2o::k_sweep<4,256,2,0>": 2680.81 * 1024 * 2.0 + 4.56017 * 1024 * 1.0}
+PROFILED_TRAFFIC = {"l2o::k_sweep<4,256,2,0>": 2680.17 * 1024 * 2.0 + 4.41877 * 1024 * 1.0}
 REF_FLOP_PER_EVAL = 36    # the reference's GPU GFLOPS convention (solver.c:305-306)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # The sweep's own bound: VALU instruction issue.  Each of the 1,024 SIMDs
@@ -60,7 +60,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # FP32).  VALU instructions per candidate come from SQ_INSTS_VALU of the
 # profiled kernel (per dispatch / candidates per dispatch, wave instructions).
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
-PROFILED_VALU_PER_EVAL = {"l2o::k_sweep<4,256,2,0>": 1.96036e7 / 571473528}
+PROFILED_VALU_PER_EVAL = {"l2o::k_sweep<4,256,2,0>": 1.96806e7 / 571473528}
 
 
 def log(*a):
