@@ -494,8 +494,9 @@ def main():
         "metric": METRIC, "value": evals / dt, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "i32", "data": "synthetic",
-        "dtype_note": "the sweep's hot loop is int32 VALU (v_sad_u32 L1 bounds); FP32 sqrt re-checks the rare "
-                      "survivors and FP64 + integer fix-up gives their exact lengths; lengths are exact integers",
+        "dtype_note": "the sweep's hot loop is integer VALU (v_sad_u8 on 8-bit coordinates for every candidate, "
+                      "v_sad_u32 L1 bounds for the blocks that pass); FP32 sqrt re-checks the rare survivors and "
+                      "FP64 + integer fix-up gives their exact lengths; lengths are exact integers",
         "config": {"workload": "33,810 clustered cities (60 Gaussian clusters, sigma 2% of side, + 10% uniform "
                                "noise) in [0,1e6)^2, greedy NN start, best-improvement 2-opt to a local optimum",
                    "n": n, "seed": 2, "moves": out["moves"], "sweeps": sweeps_per, "length": out["length"],
