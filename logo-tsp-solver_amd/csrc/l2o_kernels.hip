@@ -262,6 +262,9 @@ __device__ __forceinline__ int wave_max_i32(int v)
 #ifndef L2O_TIER0
 #define L2O_TIER0 1
 #endif
+#ifndef L2O_COOP
+#define L2O_COOP 1
+#endif
 #define L2O_T0_BIAS 0x10000u
 #ifndef L2O_WPE6_MIN_K
 #define L2O_WPE6_MIN_K 6
@@ -314,15 +317,79 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     // requested before the columns (their addresses do not depend on them);
     // indices are clamped to a0 + t_end <= n - 3
     constexpr bool QROWS = PRUNE || TIER0;
+    // COOP (the pruned sweep): the wave's points arrive as contiguous 16-byte
+    // loads (one cache line serves 8 lanes, instead of a lane stride of UF or
+    // K points) into its LDS stage, and each lane reads its own points back
+    // from there: the rows a0 + min(i, tc) for i < 64 MR, then the columns
+    // min(bw + i, n-1) for i < 64 MC.  Both segments are requested at once.
+    // The stage belongs to this wave alone and later holds its tier-0 block
+    // records.  Most pruned waves end after the row test, so the prologue is
+    // most of their life: the config-2 pruned search took 0.220 s instead of
+    // 0.245 s.  The full sweep keeps per-lane loads (0.296 s with the stage
+    // against 0.284 s: its waves live long, and the stage takes 10 KB of LDS
+    // per two-wave workgroup instead of 4 KB).  K <= 4 only: the stage grows
+    // with 64 K.
+    constexpr bool COOP = L2O_COOP && K <= 4 && PRUNE;
+    constexpr int MR = (T + 1 + 63) / 64, MC = (64 * K + 1 + 63) / 64;
+    constexpr int STG = COOP ? 64 * (MR > MC ? MR : MC) : 0;  // uint4 per wave
+    constexpr int T0W = 2 * T;                                  // tier 0: words per wave (8 per block)
+    constexpr int T0U = TIER0 ? T0W / 4 : 0;
+    constexpr int LW = STG > T0U ? STG : (T0U > 0 ? T0U : 1);
+    __shared__ __attribute__((aligned(16))) uint4 wlds[WAVES * LW];
+    uint4 *const wst = wlds + w * LW;
+    const int tc = max(t_end, 0);
+    // Lanes read slots that other lanes wrote: to the compiler, which sees one
+    // lane, those addresses are unrelated, so each phase change of the stage
+    // is a wave barrier plus a memory clobber that keeps the LDS operations
+    // in program order (the hardware then completes them in order).
+    auto stage_fence = [&]() __attribute__((always_inline)) {
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+    };
+    auto ld4 = [&](int k) __attribute__((always_inline)) {
+        return make_uint4((uint32_t)P[k].x, (uint32_t)P[k].y, (uint32_t)P[k].d, (uint32_t)P[k].w);
+    };
+    uint4 gr[COOP && QROWS ? MR : 1], gc[COOP ? MC : 1];
+    if constexpr (COOP) {
+        if constexpr (QROWS) {
+#pragma unroll
+            for (int m = 0; m < MR; ++m) gr[m] = ld4(a0 + min(lane + 64 * m, tc));
+        }
+#pragma unroll
+        for (int m = 0; m < MC; ++m) gc[m] = ld4(min(bw + lane + 64 * m, n - 1));
+    }
     int qx[QROWS ? UF + 1 : 1], qy[QROWS ? UF + 1 : 1], qw[QROWS ? UF + 1 : 1];
     if constexpr (QROWS) {
+        if constexpr (COOP) {
 #pragma unroll
-        for (int s = 0; s <= UF; ++s) {
-            const uint32_t q = (uint32_t)min(a0 + lane * UF + s, a0 + max(t_end, 0));
-            qx[s] = P[q].x;
-            qy[s] = P[q].y;
-            qw[s] = P[q].w;
+            for (int m = 0; m < MR; ++m) wst[lane + 64 * m] = gr[m];
+            stage_fence();
+#pragma unroll
+            for (int s = 0; s <= UF; ++s) {
+                const uint4 v = wst[min(lane * UF + s, tc)];
+                qx[s] = (int)v.x;
+                qy[s] = (int)v.y;
+                qw[s] = (int)v.w;
+            }
+            stage_fence();  // (the column stage below reuses these slots)
+        } else {
+#pragma unroll
+            for (int s = 0; s <= UF; ++s) {
+                const uint32_t q = (uint32_t)min(a0 + lane * UF + s, a0 + max(t_end, 0));
+                qx[s] = P[q].x;
+                qy[s] = P[q].y;
+                qw[s] = P[q].w;
+            }
         }
+    }
+    uint4 cp[COOP ? K + 1 : 1];
+    if constexpr (COOP) {
+#pragma unroll
+        for (int m = 0; m < MC; ++m) wst[lane + 64 * m] = gc[m];
+        stage_fence();
+#pragma unroll
+        for (int j = 0; j <= K; ++j) cp[j] = wst[lane * K + j];
+        stage_fence();  // (the tier-0 block records below reuse these slots)
     }
 
     // lane columns and offsets.  Wn[j] = W of column bl+j, minus 2^30 when
@@ -335,7 +402,7 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
         uint32_t Wn[K + 1];
 #pragma unroll
         for (int j = 0; j <= K; ++j)
-            Wn[j] = (uint32_t)P[min(bl + j, n - 1)].w - (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
+            Wn[j] = (COOP ? cp[j].w : (uint32_t)P[min(bl + j, n - 1)].w) - (bl + j - 1 > n - 2 ? L2O_T1_BIG : 0u);
         // (beyond n-2 Wn is negative as an int and drops out)
         if constexpr (PRUNE) {
 #pragma unroll
@@ -369,9 +436,14 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     }
 #pragma unroll
     for (int j = 0; j <= K; ++j) {
-        const uint32_t c = (uint32_t)min(bl + j, n - 1);
-        cx[j] = (uint32_t)P[c].x;
-        cy[j] = (uint32_t)P[c].y;
+        if constexpr (COOP) {
+            cx[j] = cp[j].x;
+            cy[j] = cp[j].y;
+        } else {
+            const uint32_t c = (uint32_t)min(bl + j, n - 1);
+            cx[j] = (uint32_t)P[c].x;
+            cy[j] = (uint32_t)P[c].y;
+        }
     }
     // PRUNE: the wave's bounding box [bx0, bx1] x [by0, by1] of its column points
     int bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
@@ -396,11 +468,9 @@ __device__ __forceinline__ void sweep_item(const l2o_sweep_item it, l2o_inst *__
     // the block's largest wa = ceil(W[a+1] / 2^qs), for the whole blocks (a+1
     // <= a0 + t_end <= n-3).  Only this wave reads its blocks, and LDS
     // operations of one wave complete in order: no barrier.
-    constexpr int T0W = 2 * T;  // LDS words per wave (8 per block)
     uint32_t cpk[TIER0 ? K : 1];
     uint32_t c0 = 0;
-    __shared__ __attribute__((aligned(16))) uint32_t t0lds[TIER0 ? WAVES * T0W : 1];
-    uint4 *const t0blk = (uint4 *)(t0lds + (TIER0 ? w * T0W : 0));  // 2 per block
+    uint4 *const t0blk = wst;  // 2 per block
     auto t0_pack = [&](uint32_t xa, uint32_t ya, uint32_t xb, uint32_t yb) __attribute__((always_inline)) {
         return (xa >> qs) | ((ya >> qs) << 8) | ((xb >> qs) << 16) | ((yb >> qs) << 24);
     };
