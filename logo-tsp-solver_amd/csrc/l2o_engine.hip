@@ -430,7 +430,9 @@ void drop_graph(logo2opt_engine *e)
 // pass it, go straight to tier 1 (measured: 33,810 and 100,000 cities 3-5%
 // faster without tier 0, the 2,048-instance batch of 100-5,000 cities 7%
 // faster with it)
+#ifndef L2O_PRUNE_T0_MAX_N
 #define L2O_PRUNE_T0_MAX_N 16000u
+#endif
 
 #ifndef L2O_DIAG_WEIGHT
 #define L2O_DIAG_WEIGHT 4
