@@ -28,6 +28,9 @@ Also reported on the same line:
                 the same local search, batch and serving replay with the sweep's
                 row pruning on: separate figures (fewer candidates evaluated,
                 identical results, asserted), never the headline
+  serving_order the serving replay at 50 and 100 req/s with at most 4 running
+                requests, admitted FIFO and fewest-cities-first (a request
+                overtaken 16 times goes next); p50 ratio FIFO / shortest-first
   ils_100k      500 double-bridge ILS rounds on 100,000 uniform cities from the
                 NN tour, full and pruned sweeps (--skip-config3 leaves it out)
 """
@@ -227,9 +230,11 @@ def batched_vs_plain(count, plain_every, seed=4, prune=True):
 
 
 def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=512, max_running=0,
-            max_inflight_pairs=0, prune=False):
+            max_inflight_pairs=0, prune=False, order=None):
     """Open-loop Poisson replay through the continuous-batching server (the
-    same seeded pool and arrivals for every call with the same arguments)."""
+    same seeded pool and arrivals for every call with the same arguments).
+    order: None admits in arrival order; an integer admits the fewest cities
+    first with that overtaking bound (max_bypass, 0 = none)."""
     rs = np.random.default_rng(seed)
     insts, tcon = [], []
     for k in range(pool):
@@ -250,6 +255,8 @@ def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=51
         with L.Server(hbm_budget=8 << 30, max_slots=max_slots, queue_cap=1 << 20) as srv:
             srv.set_limits(max_running, max_inflight_pairs)
             srv.set_pruning(prune)
+            if order is not None:
+                srv.set_order(True, max_bypass=order)
             lat, qs, el = srv.replay(arrivals, reqs, horizon_s, steps=32, sync_every=16)
             cnt = srv.counts()
         ok = lat >= 0
@@ -288,6 +295,7 @@ def serving(rates, window_s, horizon_s, pool=48, seed=5, rounds=50, max_slots=51
                               "counts with (end - arrival), a lower bound (flagged *_is_lower_bound when such a "
                               "value reaches the percentile); *_completed_s are over completed requests only",
             "max_slots": max_slots, "max_running": max_running, "max_inflight_pairs": max_inflight_pairs,
+            "order": "fifo" if order is None else f"shortest-first, max_bypass {order}", "pruned": bool(prune),
             "knee_req_per_s": max(keep) if keep else None,
             "saturation_throughput_req_per_s": max(r["throughput_req_per_s"] for r in out) if out else None,
             "rates": out}
@@ -398,6 +406,11 @@ def main():
     ap.add_argument("--serve-max-pairs", type=float, default=0, help="cap on in-flight candidates per sweep")
     ap.add_argument("--no-serve-prune", action="store_true",
                     help="skip the replay of the same arrivals with row pruning on (serving_pruned)")
+    ap.add_argument("--skip-order", action="store_true",
+                    help="skip the FIFO vs shortest-first admission comparison (serving_order)")
+    ap.add_argument("--order-rates", default="50,100")
+    ap.add_argument("--order-max-running", type=int, default=4)
+    ap.add_argument("--order-bypass", type=int, default=16)
     ap.add_argument("--skip-tsplib", action="store_true")
     ap.add_argument("--skip-pruned", action="store_true")
     ap.add_argument("--tsplib-chains", type=int, default=16)
@@ -521,6 +534,19 @@ def main():
             res["serving_pruned"] = serving(rates, args.serve_window, args.serve_horizon,
                                             max_running=args.serve_max_running,
                                             max_inflight_pairs=int(args.serve_max_pairs), prune=True)
+    if not args.skip_serve and not args.skip_order:
+        # admission order under an admission cap: the same arrivals admitted
+        # FIFO and fewest-cities-first (bounded overtaking)
+        log("serving, FIFO vs shortest-first under a cap on running requests")
+        orates = [float(v) for v in args.order_rates.split(",")]
+        kw = dict(max_running=args.order_max_running, prune=not args.no_serve_prune)
+        fifo = serving(orates, args.serve_window, args.serve_horizon, **kw)
+        sjf = serving(orates, args.serve_window, args.serve_horizon, order=args.order_bypass, **kw)
+        res["serving_order"] = {
+            "fifo": fifo, "shortest_first": sjf,
+            "p50_ratio_fifo_over_shortest": [a["p50_latency_s"] / b["p50_latency_s"]
+                                             for a, b in zip(fifo["rates"], sjf["rates"])],
+        }
     if not args.skip_tsplib:
         log("TSPLIB pla33810")
         res["tsplib"] = tsplib_pla33810(args.tsplib_chains, args.tsplib_rounds)

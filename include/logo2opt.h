@@ -340,6 +340,18 @@ int logo2opt_server_release(logo2opt_server *s, int64_t id);
  * summed over running requests (the head of the queue is always admitted
  * when nothing runs). */
 int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_t max_inflight_pairs);
+/* Admission order.  LOGO2OPT_ORDER_FIFO (the default) admits in arrival
+ * order, and a head that does not fit holds back everything behind it.
+ * LOGO2OPT_ORDER_SHORTEST admits the queued request with the fewest cities
+ * whose start tour is ready (earlier arrival on ties); a request that later
+ * arrivals have overtaken max_bypass times is admitted next (0 = no bound: a
+ * steady stream of smaller requests can then hold a large one back for as
+ * long as it lasts).  The order matters only when admission is limited (by
+ * logo2opt_server_set_limits, the slots or the HBM budget); results do not
+ * depend on it.  The reference has no serving layer and no counterpart. */
+#define LOGO2OPT_ORDER_FIFO 0
+#define LOGO2OPT_ORDER_SHORTEST 1
+int logo2opt_server_set_order(logo2opt_server *s, int order, uint32_t max_bypass);
 /* Row pruning for the requests' sweeps; mode as for
  * logo2opt_engine_set_pruning (LOGO2OPT_PRUNE_AUTO suits mixed sizes). */
 int logo2opt_server_set_pruning(logo2opt_server *s, int mode);

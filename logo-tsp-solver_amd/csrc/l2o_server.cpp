@@ -7,8 +7,12 @@
 // bounded number of batched steps in which every admitted request advances
 // together, and harvests the requests that finished.  A request that arrives
 // while others are on the GPU therefore joins the very next batch of steps.
-// Admission is FIFO: when the head of the queue does not fit, later requests
-// wait too (no overtaking), so latency order follows arrival order.
+// Admission is FIFO by default: when the head of the queue does not fit, later
+// requests wait too (no overtaking), so latency order follows arrival order.
+// The shortest-first order (logo2opt_server_set_order) admits the queued
+// request with the fewest cities whose start tour is ready; a request that
+// later arrivals have overtaken max_bypass times is admitted next, so no
+// request waits behind an unbounded stream of smaller ones.
 //
 // The reference has no serving layer; its closest relative is the multi-start
 // ILS driver with one solver per pthread (src/solvers/ILSGlobalSolverMT.cpp:86-195).
@@ -50,6 +54,7 @@ struct request {
     int state = Q_QUEUED;
     int err = LOGO2OPT_OK;  // Q_REJECTED: why
     double t_construct = 0;  // seconds the server spent building its start tour
+    uint32_t bypassed = 0;   // later arrivals admitted before it (shortest-first order)
     // no start tour given: the server's builder thread makes the greedy NN
     // tour while the GPU runs other requests; admission waits for it
     std::shared_ptr<build_job> build;
@@ -106,6 +111,10 @@ struct logo2opt_server {
     uint32_t max_running = 0;
     uint64_t max_inflight_pairs = 0;
     uint64_t inflight_pairs = 0;
+    // admission order: LOGO2OPT_ORDER_FIFO or LOGO2OPT_ORDER_SHORTEST (with
+    // its overtaking bound, 0 = none)
+    int order = LOGO2OPT_ORDER_FIFO;
+    uint32_t max_bypass = 0;
     int64_t next_id = 1;
     std::deque<int64_t> queue;
     std::map<int64_t, request> reqs;
@@ -119,6 +128,34 @@ struct logo2opt_server {
     std::condition_variable bcv;
     std::deque<std::shared_ptr<build_job>> bq;
     bool bstop = false;
+
+    // Queue position to admit next, or -1 when it has to wait (FIFO: the
+    // head's start tour is not built yet; shortest-first: no start tour is).
+    // Ties in size go to the earlier arrival.
+    ptrdiff_t pick()
+    {
+        const request &head = reqs[queue.front()];
+        const bool head_ready = !head.build || head.build->ready.load(std::memory_order_acquire);
+        if (order == LOGO2OPT_ORDER_FIFO || (max_bypass && head.bypassed >= max_bypass))
+            return head_ready ? 0 : -1;
+        ptrdiff_t best = -1;
+        uint32_t best_n = 0;
+        for (size_t q = 0; q < queue.size(); ++q) {
+            const request &r = reqs[queue[q]];
+            if (r.build && !r.build->ready.load(std::memory_order_acquire)) continue;
+            if (best < 0 || r.n < best_n) {
+                best = (ptrdiff_t)q;
+                best_n = r.n;
+            }
+        }
+        return best;
+    }
+    // remove queue position q; the requests ahead of it were overtaken
+    void take(ptrdiff_t q)
+    {
+        for (ptrdiff_t k = 0; k < q; ++k) reqs[queue[k]].bypassed++;
+        queue.erase(queue.begin() + q);
+    }
 
     void build_loop()
     {
@@ -214,23 +251,24 @@ int logo2opt_server_pump(logo2opt_server *s, uint32_t steps, uint32_t sync_every
 {
     if (!s) return LOGO2OPT_EINVAL;
     while (!s->queue.empty()) {
-        request &r = s->reqs[s->queue.front()];
-        const uint64_t pairs = r.n >= 4 ? (uint64_t)(r.n - 2) * (r.n - 3) / 2 : 0;
         if (s->max_running && s->slot_req.size() >= s->max_running) break;
+        const ptrdiff_t pos = s->pick();
+        if (pos < 0) break;  // its start tour is not built yet
+        request &r = s->reqs[s->queue[pos]];
+        const uint64_t pairs = r.n >= 4 ? (uint64_t)(r.n - 2) * (r.n - 3) / 2 : 0;
         if (s->max_inflight_pairs && !s->slot_req.empty() && s->inflight_pairs + pairs > s->max_inflight_pairs)
             break;
         int32_t slot = -1;
         int rc = LOGO2OPT_OK;
         if (r.build) {
-            if (!r.build->ready.load(std::memory_order_acquire)) break;  // head's start tour not built yet
             rc = r.build->rc;
             r.route.swap(r.build->route);
             r.t_construct = r.build->seconds;
             r.build.reset();
         }
         if (rc == LOGO2OPT_OK) rc = logo2opt_engine_add(s->e, r.n, r.coords.data(), r.route.data(), &r.job, &slot);
-        if (rc == LOGO2OPT_EAGAIN) break;  // head of line does not fit yet
-        s->queue.pop_front();
+        if (rc == LOGO2OPT_EAGAIN) break;  // the pick does not fit yet
+        s->take(pos);
         if (rc) {  // invalid, or larger than the whole budget: finished with an error
             r.state = Q_REJECTED;
             r.err = rc;
@@ -337,6 +375,14 @@ int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_
     if (!s) return LOGO2OPT_EINVAL;
     s->max_running = max_running;
     s->max_inflight_pairs = max_inflight_pairs;
+    return LOGO2OPT_OK;
+}
+
+int logo2opt_server_set_order(logo2opt_server *s, int order, uint32_t max_bypass)
+{
+    if (!s || (order != LOGO2OPT_ORDER_FIFO && order != LOGO2OPT_ORDER_SHORTEST)) return LOGO2OPT_EINVAL;
+    s->order = order;
+    s->max_bypass = max_bypass;
     return LOGO2OPT_OK;
 }
 

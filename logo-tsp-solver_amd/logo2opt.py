@@ -21,6 +21,7 @@ METRIC_CEIL_2D = 1
 METRIC_ATT = 2
 METRICS = {"EUC_2D": METRIC_EUC_2D, "CEIL_2D": METRIC_CEIL_2D, "ATT": METRIC_ATT}
 PRUNE_OFF, PRUNE_ALL, PRUNE_AUTO = 0, 1, 2
+ORDER_FIFO, ORDER_SHORTEST = 0, 1  # LOGO2OPT_ORDER_FIFO / LOGO2OPT_ORDER_SHORTEST
 
 
 def _prune_mode(on):
@@ -534,7 +535,7 @@ class Engine:
 SERVER_EXPORTS = ["logo2opt_server_create", "logo2opt_server_destroy", "logo2opt_server_submit",
                   "logo2opt_server_pump", "logo2opt_server_poll", "logo2opt_server_result",
                   "logo2opt_server_release", "logo2opt_server_counts", "logo2opt_server_replay",
-                  "logo2opt_server_set_limits", "logo2opt_server_set_pruning"]
+                  "logo2opt_server_set_limits", "logo2opt_server_set_pruning", "logo2opt_server_set_order"]
 EXPORTS += SERVER_EXPORTS
 
 
@@ -552,6 +553,7 @@ def _bind_server(L):
     L.logo2opt_server_counts.argtypes = [vp, u64p]
     L.logo2opt_server_set_limits.argtypes = [vp, C.c_uint32, C.c_uint64]
     L.logo2opt_server_set_pruning.argtypes = [vp, C.c_int]
+    L.logo2opt_server_set_order.argtypes = [vp, C.c_int, C.c_uint32]
     L.logo2opt_server_replay.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_double, C.c_uint32, C.c_uint32,
                                          vp, vp, C.POINTER(C.c_double)]
     L._server_bound = True
@@ -627,6 +629,12 @@ class Server:
 
     def set_pruning(self, on):
         _check(load().logo2opt_server_set_pruning(self.h, _prune_mode(on)), "server_set_pruning")
+
+    def set_order(self, shortest_first, max_bypass=0):
+        """Admission order: arrival order (False) or fewest cities first
+        (True), a request overtaken max_bypass times going next (0: no bound)."""
+        _check(load().logo2opt_server_set_order(self.h, ORDER_SHORTEST if shortest_first else ORDER_FIFO,
+                                                max_bypass), "server_set_order")
 
     def counts(self):
         c = np.zeros(6, np.uint64)

@@ -95,6 +95,49 @@ def test_server_admission_by_hbm_budget_queues_the_rest():
             assert srv.result(rid, rq["n"])["length"] == best
 
 
+def _serve_one_at_a_time(reqs, order):
+    """Submit every request, then serve them one at a time (max_running 1), so
+    completions come in admission order.  order: None (FIFO) or max_bypass for
+    shortest-first.  Returns the request indices in completion order and the
+    results by index."""
+    with L.Server(hbm_budget=256 << 20, max_slots=8) as srv:
+        srv.set_limits(max_running=1)
+        if order is not None:
+            srv.set_order(True, max_bypass=order)
+        ids = [srv.submit(rq["coords"], rq["route"], ils_rounds=rq["rounds"], seed=rq["seed"]) for rq in reqs]
+        pos = {rid: q for q, rid in enumerate(ids)}
+        seen = []
+        while True:
+            running, queued = srv.pump(steps=64, sync_every=16)
+            assert running <= 1
+            seen += [pos[rid] for rid in srv.poll()]
+            if running == 0 and queued == 0:
+                break
+        return seen, {q: srv.result(ids[q], rq["n"]) for q, rq in enumerate(reqs)}
+
+
+def test_server_shortest_first_order_and_bypass_bound():
+    sizes = [2400, 300, 1200, 150, 600]
+    reqs = []
+    for k, n in enumerate(sizes):
+        xy = L.gen_uniform(n, 10**6, 300 + k)
+        reqs.append(dict(n=n, xy=xy, coords=L.coords_of(xy), route=L.nn_tour(xy, 0), seed=2000 + k, rounds=2))
+    fifo, res_fifo = _serve_one_at_a_time(reqs, None)
+    assert fifo == [0, 1, 2, 3, 4]
+    sjf, res_sjf = _serve_one_at_a_time(reqs, 0)
+    assert [sizes[q] for q in sjf] == sorted(sizes)
+    # bound 2: the 2,400-city head is overtaken by 150 and 300, then goes next;
+    # 1,200 is then overtaken by 600 (its second time) and goes last
+    bounded, res_bounded = _serve_one_at_a_time(reqs, 2)
+    assert [sizes[q] for q in bounded] == [150, 300, 2400, 600, 1200]
+    for q, rq in enumerate(reqs):
+        r, best = alone(rq)
+        for res in (res_fifo, res_sjf, res_bounded):
+            assert res[q]["length"] == best and res[q]["route"].tolist() == r.tolist()
+    with L.Server(hbm_budget=16 << 20, max_slots=2) as srv:
+        assert L.load().logo2opt_server_set_order(srv.h, 7, 0) == -1  # LOGO2OPT_EINVAL
+
+
 def test_server_queue_cap_rejects():
     reqs = make_requests(5, 200, 300, 17, rounds=1)
     with L.Server(hbm_budget=64 << 20, max_slots=4, queue_cap=2) as srv:
