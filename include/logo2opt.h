@@ -342,8 +342,9 @@ int logo2opt_server_release(logo2opt_server *s, int64_t id);
 int logo2opt_server_set_limits(logo2opt_server *s, uint32_t max_running, uint64_t max_inflight_pairs);
 /* Admission order.  LOGO2OPT_ORDER_FIFO (the default) admits in arrival
  * order, and a head that does not fit holds back everything behind it.
- * LOGO2OPT_ORDER_SHORTEST admits the queued request with the fewest cities
- * whose start tour is ready (earlier arrival on ties); a request that later
+ * LOGO2OPT_ORDER_SHORTEST admits the request with the fewest cities whose
+ * start tour is ready among the first 1,024 queued (earlier arrival on ties,
+ * so an admission costs at most 1,024 lookups); a request that later
  * arrivals have overtaken max_bypass times is admitted next (0 = no bound: a
  * steady stream of smaller requests can then hold a large one back for as
  * long as it lasts).  The order matters only when admission is limited (by

@@ -9,10 +9,10 @@
 // while others are on the GPU therefore joins the very next batch of steps.
 // Admission is FIFO by default: when the head of the queue does not fit, later
 // requests wait too (no overtaking), so latency order follows arrival order.
-// The shortest-first order (logo2opt_server_set_order) admits the queued
-// request with the fewest cities whose start tour is ready; a request that
-// later arrivals have overtaken max_bypass times is admitted next, so no
-// request waits behind an unbounded stream of smaller ones.
+// The shortest-first order (logo2opt_server_set_order) admits the request with
+// the fewest cities whose start tour is ready among the first 1,024 queued; a
+// request that later arrivals have overtaken max_bypass times is admitted
+// next, so no request waits behind an unbounded stream of smaller ones.
 //
 // The reference has no serving layer; its closest relative is the multi-start
 // ILS driver with one solver per pthread (src/solvers/ILSGlobalSolverMT.cpp:86-195).
@@ -131,7 +131,10 @@ struct logo2opt_server {
 
     // Queue position to admit next, or -1 when it has to wait (FIFO: the
     // head's start tour is not built yet; shortest-first: no start tour is).
+    // Shortest-first looks at the first ORDER_WINDOW queued requests, so an
+    // admission costs at most that many lookups however long the queue is.
     // Ties in size go to the earlier arrival.
+    static constexpr size_t ORDER_WINDOW = 1024;
     ptrdiff_t pick()
     {
         const request &head = reqs[queue.front()];
@@ -140,7 +143,7 @@ struct logo2opt_server {
             return head_ready ? 0 : -1;
         ptrdiff_t best = -1;
         uint32_t best_n = 0;
-        for (size_t q = 0; q < queue.size(); ++q) {
+        for (size_t q = 0; q < std::min(queue.size(), ORDER_WINDOW); ++q) {
             const request &r = reqs[queue[q]];
             if (r.build && !r.build->ready.load(std::memory_order_acquire)) continue;
             if (best < 0 || r.n < best_n) {
