@@ -5,7 +5,8 @@ uniform or clustered points, random or nearest-neighbour starts, with row
 pruning on or off for the whole engine.  Instances of different sizes,
 metrics and job kinds share every launch, so the per-instance state, the
 work lists, the ILS bookkeeping and the group barrier are all exercised
-together.  LOGO2OPT_RANDOM_ENGINE_JOBS sets the number of jobs (default 24)."""
+together.  LOGO2OPT_RANDOM_ENGINE_JOBS sets the number of jobs (default 24),
+LOGO2OPT_RANDOM_ENGINE_SEED adds to the seeds that draw them (default 0)."""
 import os
 import sys
 
@@ -18,6 +19,7 @@ import oracle_lib as O
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
 JOBS = int(os.environ.get("LOGO2OPT_RANDOM_ENGINE_JOBS", "24"))
+SEED = int(os.environ.get("LOGO2OPT_RANDOM_ENGINE_SEED", "0"))
 
 
 def say(*a):
@@ -32,9 +34,10 @@ def jobs(count, seed):
         n = int(np.rint(np.exp(rs.uniform(np.log(5), np.log(1200)))))
         side = int(rs.choice([100, 10**4, 10**6]))
         metric = int(rs.integers(0, 3))
-        xy = O.gen_clustered(n, side, 300 + q) if (side >= 10**4 and rs.random() < 0.5) else \
-            O.gen_uniform(n, side, 300 + q)
-        r0 = O.nn_tour(xy, 0) if rs.random() < 0.5 else O.random_perm(n, 500 + q)
+        base = 100003 * SEED  # 0 by default: the default jobs' point and start-tour seeds
+        xy = O.gen_clustered(n, side, base + 300 + q) if (side >= 10**4 and rs.random() < 0.5) else \
+            O.gen_uniform(n, side, base + 300 + q)
+        r0 = O.nn_tour(xy, 0) if rs.random() < 0.5 else O.random_perm(n, base + 500 + q)
         kind = str(rs.choice(["lo", "ils", "group"]))
         if n < 8:  # ILS needs n >= 8 (three double-bridge cuts); the engine rejects less
             kind = "lo"
@@ -48,7 +51,7 @@ def jobs(count, seed):
 
 @pytest.mark.parametrize("prune", [False, True], ids=["full", "pruned"])
 def test_random_engine_mix_matches_oracle(prune):
-    js = jobs(JOBS, 77 if prune else 76)
+    js = jobs(JOBS, (77 if prune else 76) + 1000 * SEED)
     say(f"engine mix: {len(js)} jobs, prune={prune}, kinds", [j["kind"] for j in js])
     with L.Engine(hbm_budget=1 << 29, max_slots=256) as e:
         e.set_pruning(prune)
