@@ -53,7 +53,9 @@ def jobs(count, seed):
 def test_random_engine_mix_matches_oracle(prune):
     js = jobs(JOBS, (77 if prune else 76) + 1000 * SEED)
     say(f"engine mix: {len(js)} jobs, prune={prune}, kinds", [j["kind"] for j in js])
-    with L.Engine(hbm_budget=1 << 29, max_slots=256) as e:
+    # a multi-start group takes one slot per chain
+    slots = sum(j["chains"] if j["kind"] == "group" else 1 for j in js)
+    with L.Engine(hbm_budget=1 << 29 if slots <= 256 else 1 << 31, max_slots=max(256, slots)) as e:
         e.set_pruning(prune)
         for j in js:
             c = L.coords_of(j["xy"])
